@@ -1,0 +1,280 @@
+"""dtwax_amd -- MI355X (gfx950) subsequence-DTW read classifier.
+
+Python host-side mirror of the C ABI in include/dtwax.h (ctypes over the
+in-tree ``libdtwax.so``).  Device memory is owned by the library's own HIP
+calls (no torch).  If the library is missing the import of any entry point
+raises: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdtwax.so")
+
+F16, F16_FUSED, F16_RENORM, F32 = 0, 1, 2, 3
+PRECISIONS = {"f16": F16, "f16_fused": F16_FUSED, "f16_renorm": F16_RENORM, "f32": F32}
+NORM_NONE, NORM_MEANSTD, NORM_MEDMAD = 0, 1, 2
+
+_lib = None
+
+
+class DtwaxError(RuntimeError):
+    pass
+
+
+class ScoreOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("segment", C.c_int32), ("ref_chunk", C.c_int32),
+                ("renorm_drift", C.c_float), ("compat_quirks", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
+
+
+RESULT_DTYPE = np.dtype([("score", np.float32, 2), ("argmin", np.int32, 2)])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DtwaxError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C "
+                             "dtwax_amd/csrc); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+        sig = {
+            "dtwax_last_error": ([], C.c_char_p),
+            "dtwax_version": ([], C.c_char_p),
+            "dtwax_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+            "dtwax_destroy": ([vp], None),
+            "dtwax_device_info": ([vp, C.POINTER(i32), C.POINTER(i32), C.c_char_p, i32], C.c_int),
+            "dtwax_malloc": ([C.POINTER(vp), C.c_size_t], C.c_int),
+            "dtwax_free": ([vp], C.c_int),
+            "dtwax_memcpy_h2d": ([vp, vp, C.c_size_t, vp], C.c_int),
+            "dtwax_memcpy_d2h": ([vp, vp, C.c_size_t, vp], C.c_int),
+            "dtwax_stream_create": ([C.POINTER(vp)], C.c_int),
+            "dtwax_stream_destroy": ([vp], C.c_int),
+            "dtwax_stream_sync": ([vp], C.c_int),
+            "dtwax_device_sync": ([], C.c_int),
+            "dtwax_event_create": ([C.POINTER(vp)], C.c_int),
+            "dtwax_event_destroy": ([vp], C.c_int),
+            "dtwax_event_record": ([vp, vp], C.c_int),
+            "dtwax_event_elapsed_ms": ([vp, vp, C.POINTER(f32)], C.c_int),
+            "dtwax_reference_upload": ([vp, vp, i32, i64], C.c_int),
+            "dtwax_squiggle_build": ([C.c_char_p, i64, i32, vp, i64, vp, C.POINTER(i64)], C.c_int),
+            "dtwax_normalize": ([vp, vp, vp, vp, i32, i32, C.c_double, vp, vp, vp], C.c_int),
+            "dtwax_score": ([vp, vp, vp, vp, i32, vp, vp, vp], C.c_int),
+            "dtwax_plan_create": ([vp, vp, i32, vp, C.POINTER(vp)], C.c_int),
+            "dtwax_plan_destroy": ([vp], None),
+            "dtwax_plan_info": ([vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32), C.POINTER(i32),
+                                 C.POINTER(i32), C.POINTER(i32)], C.c_int),
+            "dtwax_score_plan": ([vp, vp, vp, vp, vp, vp], C.c_int),
+            "dtwax_plan_set_timing": ([vp, i32], C.c_int),
+            "dtwax_plan_kernel_ms": ([vp, C.POINTER(f32)], C.c_int),
+            "dtwax_distances": ([vp, vp, vp, vp, i32, C.c_uint32, vp, vp, i32], C.c_int),
+            "dtwax_selftest_lanes": ([vp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names every C-ABI entry point in include/dtwax.h (checked by tests)."""
+    return [
+        "dtwax_last_error", "dtwax_version", "dtwax_create", "dtwax_destroy", "dtwax_device_info",
+        "dtwax_malloc", "dtwax_free", "dtwax_memcpy_h2d", "dtwax_memcpy_d2h", "dtwax_stream_create",
+        "dtwax_stream_destroy", "dtwax_stream_sync", "dtwax_device_sync", "dtwax_event_create",
+        "dtwax_event_destroy", "dtwax_event_record", "dtwax_event_elapsed_ms",
+        "dtwax_reference_upload", "dtwax_squiggle_build", "dtwax_normalize", "dtwax_score",
+        "dtwax_plan_create", "dtwax_plan_destroy", "dtwax_plan_info", "dtwax_score_plan",
+        "dtwax_plan_set_timing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
+    ]
+
+
+def _check(rc):
+    if rc != 0:
+        raise DtwaxError(f"dtwax error {rc}: {lib().dtwax_last_error().decode()}")
+
+
+class DeviceArray:
+    """A device allocation holding a numpy-shaped array."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(lib().dtwax_malloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes:
+            _check(lib().dtwax_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes, stream))
+        return d
+
+    def to_host(self, stream=None):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(lib().dtwax_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, stream))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().dtwax_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def pack_reads(reads):
+    """List of 1-D sample arrays -> (flat float32, offsets int64, lengths int32)."""
+    lens = np.array([len(r) for r in reads], np.int32)
+    offs = np.zeros(len(reads), np.int64)
+    if len(reads) > 1:
+        offs[1:] = np.cumsum(lens.astype(np.int64))[:-1]
+    flat = (np.concatenate([np.asarray(r, np.float32) for r in reads]) if len(reads)
+            else np.zeros(0, np.float32))
+    return np.ascontiguousarray(flat, np.float32), offs, lens
+
+
+def make_opts(precision=F16, segment=0, ref_chunk=0, renorm_drift=0.0):
+    if isinstance(precision, str):
+        precision = PRECISIONS[precision]
+    return ScoreOpts(int(precision), int(segment), int(ref_chunk), float(renorm_drift), 0)
+
+
+class Plan:
+    def __init__(self, ctx, lens, **kw):
+        self.ctx = ctx
+        lens = np.ascontiguousarray(lens, np.int32)
+        self.n_reads = lens.size
+        self.opts = make_opts(**kw)
+        p = C.c_void_p()
+        _check(lib().dtwax_plan_create(ctx.handle, lens.ctypes.data, lens.size,
+                                       C.byref(self.opts), C.byref(p)))
+        self.handle = p.value
+
+    def info(self):
+        cells, comp = C.c_int64(), C.c_int64()
+        n_items, seg, chunk, grid = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().dtwax_plan_info(self.handle, C.byref(cells), C.byref(comp), C.byref(n_items),
+                                     C.byref(seg), C.byref(chunk), C.byref(grid)))
+        return dict(cells=cells.value, computed=comp.value, n_items=n_items.value,
+                    segment=seg.value, ref_chunk=chunk.value, grid=grid.value)
+
+    def run(self, d_queries, d_offsets, d_lens, d_out, stream=None):
+        _check(lib().dtwax_score_plan(self.handle, d_queries.ptr, d_offsets.ptr, d_lens.ptr,
+                                      d_out.ptr, stream))
+
+    def set_timing(self, on=True):
+        _check(lib().dtwax_plan_set_timing(self.handle, 1 if on else 0))
+
+    def kernel_ms(self):
+        ms = C.c_float()
+        _check(lib().dtwax_plan_kernel_ms(self.handle, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            lib().dtwax_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device=0):
+        p = C.c_void_p()
+        _check(lib().dtwax_create(device, C.byref(p)))
+        self.handle = p.value
+        self.n_strands = 0
+        self.n_cols = 0
+
+    def close(self):
+        if self.handle:
+            lib().dtwax_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        n_cu, clk = C.c_int32(), C.c_int32()
+        name = C.create_string_buffer(64)
+        _check(lib().dtwax_device_info(self.handle, C.byref(n_cu), C.byref(clk), name, 64))
+        return dict(n_cu=n_cu.value, clock_khz=clk.value, name=name.value.decode())
+
+    def upload_reference(self, levels):
+        levels = np.ascontiguousarray(np.atleast_2d(levels), np.float32)
+        _check(lib().dtwax_reference_upload(self.handle, levels.ctypes.data, levels.shape[0],
+                                            levels.shape[1]))
+        self.n_strands, self.n_cols = levels.shape
+
+    def normalize(self, reads, mode=NORM_MEDMAD, eps=0.0, stream=None):
+        """reads: list of arrays -> list of normalised float32 arrays (via the GPU)."""
+        flat, offs, lens = pack_reads(reads)
+        d_raw, d_off, d_len = (DeviceArray.from_host(x) for x in (flat, offs, lens))
+        d_out = DeviceArray(flat.shape, np.float32)
+        d_st = DeviceArray((2 * max(1, len(reads)),), np.float64)
+        _check(lib().dtwax_normalize(self.handle, d_raw.ptr, d_off.ptr, d_len.ptr, len(reads),
+                                     mode, eps, d_out.ptr, d_st.ptr, stream))
+        _check(lib().dtwax_stream_sync(stream))
+        out = d_out.to_host()
+        st = d_st.to_host()[: 2 * len(reads)].reshape(-1, 2)
+        return [out[o:o + n] for o, n in zip(offs, lens)], st
+
+    def score(self, reads, **kw):
+        """reads: list of normalised float arrays -> (scores (n,2) f32, argmin (n,2) i32)."""
+        flat, offs, lens = pack_reads(reads)
+        d_q, d_off, d_len = (DeviceArray.from_host(x) for x in (flat, offs, lens))
+        d_out = DeviceArray((max(1, len(reads)),), RESULT_DTYPE)
+        opts = make_opts(**kw)
+        _check(lib().dtwax_score(self.handle, d_q.ptr, d_off.ptr, d_len.ptr, len(reads),
+                                 C.byref(opts), d_out.ptr, None))
+        res = d_out.to_host()[: len(reads)]
+        return res["score"].copy(), res["argmin"].copy()
+
+    def plan(self, lens, **kw):
+        return Plan(self, lens, **kw)
+
+    def distances(self, d_ref_tiled, d_query_h2, d_dists, num_entries, compat_quirks=0,
+                  stream=None):
+        _check(lib().dtwax_distances(self.handle, d_ref_tiled.ptr, d_query_h2.ptr, d_dists.ptr,
+                                     num_entries, 0, stream, None, compat_quirks))
+
+
+def squiggle_build(seq: str, model_means, ref_len=None, k=6):
+    """Host-side squiggle build (load_reference.hpp:33-108 arithmetic) -> (2, ref_len)."""
+    mm = np.ascontiguousarray(model_means, np.float32)
+    if ref_len is None:
+        ref_len = len(seq) - k + 1
+    out = np.empty(2 * ref_len, np.float32)
+    skipped = C.c_int64()
+    _check(lib().dtwax_squiggle_build(seq.encode(), len(seq), k, mm.ctypes.data, ref_len,
+                                      out.ctypes.data, C.byref(skipped)))
+    return out.reshape(2, ref_len), skipped.value
+
+
+def selftest_lanes():
+    out = np.empty(128, np.int32)
+    _check(lib().dtwax_selftest_lanes(out.ctypes.data))
+    return out[:64], out[64:]

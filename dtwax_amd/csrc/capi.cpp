@@ -1,0 +1,562 @@
+// capi.cpp -- host side of the C ABI declared in include/dtwax.h.
+//
+// Owns the device copies of the reference, plans the (read, column-chunk)
+// work items for the persistent sDTW kernel, and maps every HIP error to a
+// status code with a message (the reference instead exits on the first error
+// it sees, src/include/hpc_helpers.hpp:103-113).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dtwax.h"
+#include "kernels.h"
+
+using namespace dtwax;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(DTWAX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+inline hipStream_t S_(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+struct dtwax_ctx {
+  int device = 0;
+  int n_cu = 0;
+  int clock_khz = 0;
+  std::string name;
+  // reference
+  int32_t n_strands = 0;
+  int64_t n_cols = 0;
+  void* ref16_alloc = nullptr;  // h2 per column, COL_PADL + n + COL_PADR
+  void* ref32_alloc = nullptr;  // n_strands floats per column, same padding
+  // compat entry point buffers
+  void* compat_ref = nullptr;
+  float* compat_q = nullptr;
+  int64_t compat_q_cap = 0;
+  int64_t* compat_off = nullptr;
+  int32_t* compat_len = nullptr;
+  DeviceResult* compat_res = nullptr;
+  int32_t compat_cap = 0;
+  dtwax_plan* compat_plan = nullptr;
+  int32_t compat_plan_n = -1;
+  int32_t compat_plan_quirks = -1;
+};
+
+struct dtwax_plan {
+  dtwax_ctx* ctx = nullptr;
+  int32_t n_reads = 0;
+  int precision = DTWAX_F16;
+  int S = 32;
+  int nstr = 2;
+  int32_t chunk = 0;
+  bool inj = false;
+  int grid = 0;
+  int32_t n_items = 0;
+  int64_t cells = 0, computed = 0;
+  float renorm_k = 0.f;
+  WorkItem* d_items = nullptr;
+  int32_t* d_counter = nullptr;
+  unsigned long long* d_keys = nullptr;
+  void* d_scratch = nullptr;
+  int64_t scratch_stride = 0;
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed_once = false;
+};
+
+extern "C" {
+
+const char* dtwax_last_error(void) { return g_err.c_str(); }
+const char* dtwax_version(void) { return "dtwax-mi355x 0.1 (gfx950)"; }
+
+int dtwax_create(int device, dtwax_ctx** out) {
+  if (!out) return fail(DTWAX_EINVAL, "out is NULL");
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(DTWAX_EINVAL, "no such HIP device");
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  auto* c = new dtwax_ctx;
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  c->clock_khz = prop.clockRate;
+  c->name = prop.gcnArchName;
+  *out = c;
+  return DTWAX_OK;
+}
+
+static void free_reference(dtwax_ctx* c) {
+  if (c->ref16_alloc) (void)hipFree(c->ref16_alloc);
+  if (c->ref32_alloc) (void)hipFree(c->ref32_alloc);
+  c->ref16_alloc = c->ref32_alloc = nullptr;
+  c->n_cols = 0;
+  c->n_strands = 0;
+}
+
+void dtwax_destroy(dtwax_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  free_reference(c);
+  if (c->compat_plan) dtwax_plan_destroy(c->compat_plan);
+  for (void* p : {(void*)c->compat_ref, (void*)c->compat_q, (void*)c->compat_off, (void*)c->compat_len,
+                  (void*)c->compat_res})
+    if (p) (void)hipFree(p);
+  delete c;
+}
+
+int dtwax_device_info(dtwax_ctx* c, int32_t* n_cu, int32_t* clock_khz, char* name, int32_t name_len) {
+  if (!c) return fail(DTWAX_EINVAL, "ctx is NULL");
+  if (n_cu) *n_cu = c->n_cu;
+  if (clock_khz) *clock_khz = c->clock_khz;
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s", c->name.c_str());
+  }
+  return DTWAX_OK;
+}
+
+// ---- memory / streams / events -------------------------------------------
+int dtwax_malloc(void** p, size_t bytes) {
+  if (!p) return fail(DTWAX_EINVAL, "ptr is NULL");
+  hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+  if (e != hipSuccess) return fail(DTWAX_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return DTWAX_OK;
+}
+int dtwax_free(void* p) {
+  HIP_TRY(hipFree(p));
+  return DTWAX_OK;
+}
+int dtwax_memcpy_h2d(void* dst, const void* src, size_t bytes, void* st) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S_(st)));
+  HIP_TRY(hipStreamSynchronize(S_(st)));
+  return DTWAX_OK;
+}
+int dtwax_memcpy_d2h(void* dst, const void* src, size_t bytes, void* st) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S_(st)));
+  HIP_TRY(hipStreamSynchronize(S_(st)));
+  return DTWAX_OK;
+}
+int dtwax_stream_create(void** st) {
+  hipStream_t s;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *st = s;
+  return DTWAX_OK;
+}
+int dtwax_stream_destroy(void* st) {
+  HIP_TRY(hipStreamDestroy(S_(st)));
+  return DTWAX_OK;
+}
+int dtwax_stream_sync(void* st) {
+  HIP_TRY(hipStreamSynchronize(S_(st)));
+  return DTWAX_OK;
+}
+int dtwax_device_sync(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return DTWAX_OK;
+}
+int dtwax_event_create(void** ev) {
+  hipEvent_t e;
+  HIP_TRY(hipEventCreate(&e));
+  *ev = e;
+  return DTWAX_OK;
+}
+int dtwax_event_destroy(void* ev) {
+  HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+  return DTWAX_OK;
+}
+int dtwax_event_record(void* ev, void* st) {
+  HIP_TRY(hipEventRecord((hipEvent_t)ev, S_(st)));
+  return DTWAX_OK;
+}
+int dtwax_event_elapsed_ms(void* a, void* b, float* ms) {
+  HIP_TRY(hipEventSynchronize((hipEvent_t)b));
+  HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return DTWAX_OK;
+}
+
+// ---- reference -----------------------------------------------------------
+static const _Float16 kPadF16 = (_Float16)65504.0f;  // (pad - q)^2 overflows to +inf
+static const float kPadF32 = 1e30f;                   // (pad - q)^2 overflows to +inf
+
+int dtwax_reference_upload(dtwax_ctx* c, const float* levels, int32_t n_strands, int64_t n_cols) {
+  if (!c || !levels) return fail(DTWAX_EINVAL, "NULL argument");
+  if (n_strands < 1 || n_strands > 2) return fail(DTWAX_EINVAL, "n_strands must be 1 or 2");
+  if (n_cols < 1 || n_cols > (int64_t)INT32_MAX - 4096) return fail(DTWAX_EINVAL, "n_cols out of range");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipDeviceSynchronize());
+  free_reference(c);
+  const int64_t total = COL_PADL + n_cols + COL_PADR;
+  std::vector<uint32_t> h16((size_t)total);
+  std::vector<float> h32((size_t)(total * n_strands));
+  for (int64_t i = 0; i < total; i++) {
+    const int64_t col = i - COL_PADL;
+    const bool real = col >= 0 && col < n_cols;
+    _Float16 a = real ? (_Float16)levels[col] : kPadF16;
+    _Float16 b = (real && n_strands == 2) ? (_Float16)levels[n_cols + col] : kPadF16;
+    h16[(size_t)i] = (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+    for (int k = 0; k < n_strands; k++)
+      h32[(size_t)(i * n_strands + k)] = real ? levels[k * n_cols + col] : kPadF32;
+  }
+  hipError_t e1 = hipMalloc(&c->ref16_alloc, (size_t)total * 4);
+  hipError_t e2 = hipMalloc(&c->ref32_alloc, (size_t)total * 4 * n_strands);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    free_reference(c);
+    return fail(DTWAX_ENOMEM, "reference allocation failed");
+  }
+  HIP_TRY(hipMemcpy(c->ref16_alloc, h16.data(), (size_t)total * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->ref32_alloc, h32.data(), (size_t)total * 4 * n_strands, hipMemcpyHostToDevice));
+  c->n_cols = n_cols;
+  c->n_strands = n_strands;
+  return DTWAX_OK;
+}
+
+// ---- normalisation -------------------------------------------------------
+int dtwax_normalize(dtwax_ctx* c, const float* d_raw, const int64_t* d_off, const int32_t* d_len,
+                    int32_t n_reads, int32_t mode, double eps, float* d_out, double* d_stats, void* st) {
+  if (!c || (!d_raw && n_reads) || (!d_out && n_reads)) return fail(DTWAX_EINVAL, "NULL argument");
+  if (mode < DTWAX_NORM_NONE || mode > DTWAX_NORM_MEDMAD) return fail(DTWAX_EINVAL, "bad mode");
+  if (n_reads < 0) return fail(DTWAX_EINVAL, "n_reads < 0");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_normalize(d_raw, d_off, d_len, n_reads, mode, eps, d_out, d_stats, S_(st)));
+  return DTWAX_OK;
+}
+
+// ---- planning ------------------------------------------------------------
+static int occupancy(int precision, int S, int nstr, bool inj) {
+  return precision == DTWAX_F32 ? occupancy_f32(S, nstr, inj) : occupancy_f16(precision, S, inj);
+}
+
+static bool valid_segment(int S) { return S == 8 || S == 16 || S == 25 || S == 32; }
+
+static int auto_segment(const std::vector<int32_t>& lens) {
+  // minimise padded rows x per-step overhead (4*S cell ops + ~4 others)
+  static const int cand[] = {8, 16, 25, 32};
+  double best = 1e300;
+  int bs = 32;
+  for (int S : cand) {
+    const int64_t BR = 64 * S;
+    double rows = 0;
+    for (int32_t q : lens) rows += (double)(((int64_t)q + BR - 1) / BR * BR);
+    const double cost = rows * (4.0 * S + 4.0) / (4.0 * S);
+    if (cost < best - 1e-9) { best = cost; bs = S; }
+  }
+  return bs;
+}
+
+int dtwax_plan_create(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, const dtwax_score_opts* opts,
+                      dtwax_plan** out) {
+  if (!c || !out || (!h_lens && n_reads)) return fail(DTWAX_EINVAL, "NULL argument");
+  if (n_reads < 0) return fail(DTWAX_EINVAL, "n_reads < 0");
+  if (!c->n_cols) return fail(DTWAX_ENOTREADY, "no reference uploaded");
+  dtwax_score_opts o{};
+  if (opts) o = *opts;
+  if (o.precision < DTWAX_F16 || o.precision > DTWAX_F32) return fail(DTWAX_EINVAL, "bad precision");
+  if (o.segment && !valid_segment(o.segment)) return fail(DTWAX_EINVAL, "segment must be 8, 16, 25 or 32");
+  if (o.ref_chunk < 0) return fail(DTWAX_EINVAL, "ref_chunk < 0");
+  if (o.compat_quirks) return fail(DTWAX_EINVAL, "compat_quirks is only available through dtwax_distances");
+  std::vector<int32_t> lens(h_lens, h_lens + n_reads);
+  int32_t qmax = 0;
+  for (int32_t q : lens) {
+    if (q < 1) return fail(DTWAX_EINVAL, "every read needs at least one sample");
+    qmax = std::max(qmax, q);
+  }
+  HIP_TRY(hipSetDevice(c->device));
+
+  auto* p = new dtwax_plan;
+  p->ctx = c;
+  p->n_reads = n_reads;
+  p->precision = o.precision;
+  p->nstr = c->n_strands;
+  p->S = o.segment ? o.segment : auto_segment(lens);
+  const float drift = o.renorm_drift > 0 ? o.renorm_drift : 0.1f;
+  p->renorm_k = drift * (float)p->S;
+  const int64_t N = c->n_cols;
+  const int64_t BR = 64 * p->S;
+  bool multiband = false;
+  for (int32_t q : lens) multiband |= q > BR;
+
+  const int occ = std::max(1, occupancy(p->precision, p->S, p->nstr, false));
+  const int64_t slots = (int64_t)occ * c->n_cu;
+
+  // chunking: explicit, or whole strands unless the GPU would be underfilled
+  // or the band scratch would not fit the budget.
+  int64_t chunk = o.ref_chunk ? o.ref_chunk : N;
+  if (!o.ref_chunk) {
+    if ((int64_t)n_reads < slots && n_reads > 0) {
+      int64_t want = (slots + n_reads - 1) / n_reads;
+      int64_t ch = (N + want - 1) / want;
+      chunk = std::max<int64_t>(ch, std::min<int64_t>(N, 8 * (int64_t)qmax));
+    }
+    if (multiband) {
+      const int64_t budget = (int64_t)8 << 30;  // bytes of band scratch
+      const int64_t bytes_per_col = 4 * (p->precision == DTWAX_F32 ? p->nstr : 1);
+      const int64_t max_w = budget / (std::min<int64_t>(slots, (int64_t)n_reads * ((N + chunk - 1) / chunk)) * bytes_per_col);
+      if (chunk + qmax > max_w) chunk = std::max<int64_t>(4096, max_w - qmax);
+    }
+  }
+  chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, N));
+  p->chunk = (int32_t)chunk;
+
+  std::vector<std::pair<int64_t, WorkItem>> items;
+  int64_t max_w = 0;
+  const int strands_counted = c->n_strands;
+  for (int32_t r = 0; r < n_reads; r++) {
+    const int64_t q = lens[r];
+    const int64_t nb = (q + BR - 1) / BR;
+    for (int64_t c0 = 0; c0 < N; c0 += chunk) {
+      const int64_t c1 = std::min<int64_t>(N, c0 + chunk);
+      const int64_t x = std::max<int64_t>(0, c0 - (q - 1));
+      WorkItem it{};
+      it.read = r;
+      it.slot = 0;
+      it.x = (int32_t)x;
+      it.c0 = (int32_t)c0;
+      it.c1 = (int32_t)c1;
+      const int64_t steps = ((c1 - x + 63) + 63) / 64 * 64;
+      const int64_t cost = nb * steps;
+      items.push_back({cost, it});
+      p->cells += q * (c1 - c0) * strands_counted;
+      p->computed += nb * BR * steps * (p->precision == DTWAX_F32 ? p->nstr : 2);
+      max_w = std::max(max_w, c1 - x);
+    }
+  }
+  std::stable_sort(items.begin(), items.end(),
+                   [](const std::pair<int64_t, WorkItem>& a, const std::pair<int64_t, WorkItem>& b) {
+                     return a.first > b.first;
+                   });
+  p->n_items = (int32_t)items.size();
+  p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(slots, (int64_t)items.size()));
+
+  std::vector<WorkItem> hv;
+  hv.reserve(items.size());
+  for (auto& pr : items) hv.push_back(pr.second);
+  auto cleanup = [&](const std::string& m) {
+    dtwax_plan_destroy(p);
+    return fail(DTWAX_ENOMEM, m);
+  };
+  if (hipMalloc(&p->d_items, std::max<size_t>(1, hv.size()) * sizeof(WorkItem)) != hipSuccess)
+    return cleanup("item allocation failed");
+  if (hipMalloc(&p->d_counter, sizeof(int32_t)) != hipSuccess) return cleanup("counter allocation failed");
+  if (hipMalloc(&p->d_keys, std::max<size_t>(1, (size_t)n_reads * 2) * sizeof(unsigned long long)) != hipSuccess)
+    return cleanup("key allocation failed");
+  if (!hv.empty()) HIP_TRY(hipMemcpy(p->d_items, hv.data(), hv.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  if (multiband) {
+    const int64_t epc = (p->precision == DTWAX_F32) ? p->nstr : 1;  // 4-byte elements per column
+    p->scratch_stride = COL_PADL + max_w + COL_PADR;
+    const size_t bytes = (size_t)p->grid * (size_t)p->scratch_stride * (size_t)epc * 4;
+    if (hipMalloc(&p->d_scratch, bytes) != hipSuccess) return cleanup("band scratch allocation failed");
+    HIP_TRY(hipMemset(p->d_scratch, 0, bytes));
+  }
+  *out = p;
+  return DTWAX_OK;
+}
+
+void dtwax_plan_destroy(dtwax_plan* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipDeviceSynchronize();
+  for (void* q : {(void*)p->d_items, (void*)p->d_counter, (void*)p->d_keys, p->d_scratch})
+    if (q) (void)hipFree(q);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  delete p;
+}
+
+int dtwax_plan_info(const dtwax_plan* p, int64_t* cells, int64_t* computed, int32_t* n_items, int32_t* segment,
+                    int32_t* ref_chunk, int32_t* grid) {
+  if (!p) return fail(DTWAX_EINVAL, "plan is NULL");
+  if (cells) *cells = p->cells;
+  if (computed) *computed = p->computed;
+  if (n_items) *n_items = p->n_items;
+  if (segment) *segment = p->S;
+  if (ref_chunk) *ref_chunk = p->chunk;
+  if (grid) *grid = p->grid;
+  return DTWAX_OK;
+}
+
+int dtwax_plan_set_timing(dtwax_plan* p, int32_t enable) {
+  if (!p) return fail(DTWAX_EINVAL, "plan is NULL");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  if (enable && !p->ev0) {
+    HIP_TRY(hipEventCreate(&p->ev0));
+    HIP_TRY(hipEventCreate(&p->ev1));
+  }
+  p->timing = enable != 0;
+  return DTWAX_OK;
+}
+
+int dtwax_plan_kernel_ms(dtwax_plan* p, float* ms) {
+  if (!p || !ms) return fail(DTWAX_EINVAL, "NULL argument");
+  if (!p->timing || !p->timed_once) return fail(DTWAX_ENOTREADY, "no timed launch yet");
+  HIP_TRY(hipEventSynchronize(p->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, p->ev0, p->ev1));
+  return DTWAX_OK;
+}
+
+static int run_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const int32_t* d_len,
+                    dtwax_result* d_out, hipStream_t st, const void* row_in, void* row_out,
+                    int64_t row_stride, const WorkItem* items_override, int32_t n_items_override) {
+  dtwax_ctx* c = p->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_reset(p->d_counter, p->d_keys, (int64_t)p->n_reads * 2, st));
+  KParams k{};
+  k.ref = (p->precision == DTWAX_F32)
+              ? (const void*)((const float*)c->ref32_alloc + (int64_t)COL_PADL * c->n_strands)
+              : (const void*)((const uint32_t*)c->ref16_alloc + COL_PADL);
+  k.n_cols = (int32_t)c->n_cols;
+  k.n_items = items_override ? n_items_override : p->n_items;
+  k.queries = d_q;
+  k.q_off = d_off;
+  k.q_len = d_len;
+  k.items = items_override ? items_override : p->d_items;
+  k.counter = p->d_counter;
+  k.keys = p->d_keys;
+  k.scratch = p->d_scratch;
+  k.scratch_stride = p->scratch_stride;
+  k.row_in = row_in;
+  k.row_out = row_out;
+  k.row_stride = row_stride;
+  k.renorm_k = p->renorm_k;
+  const int grid = items_override ? std::max(1, std::min(p->grid, n_items_override)) : p->grid;
+  if (k.n_items > 0) {
+    if (p->timing) HIP_TRY(hipEventRecord(p->ev0, st));
+    HIP_TRY(launch_sdtw(k, p->precision, p->S, p->nstr, grid, p->inj, st));
+    if (p->timing) {
+      HIP_TRY(hipEventRecord(p->ev1, st));
+      p->timed_once = true;
+    }
+  }
+  HIP_TRY(launch_finalize(p->d_keys, p->n_reads, p->precision == DTWAX_F32 ? p->nstr : 2,
+                          reinterpret_cast<DeviceResult*>(d_out), st));
+  return DTWAX_OK;
+}
+
+int dtwax_score_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const int32_t* d_len,
+                     dtwax_result* d_out, void* st) {
+  if (!p || (p->n_reads && (!d_q || !d_off || !d_len || !d_out))) return fail(DTWAX_EINVAL, "NULL argument");
+  if (!p->ctx->n_cols) return fail(DTWAX_ENOTREADY, "no reference uploaded");
+  return run_plan(p, d_q, d_off, d_len, d_out, S_(st), nullptr, nullptr, 0, nullptr, 0);
+}
+
+int dtwax_score(dtwax_ctx* c, const float* d_q, const int64_t* d_off, const int32_t* d_len, int32_t n_reads,
+                const dtwax_score_opts* opts, dtwax_result* d_out, void* st) {
+  if (!c) return fail(DTWAX_EINVAL, "ctx is NULL");
+  if (n_reads < 0) return fail(DTWAX_EINVAL, "n_reads < 0");
+  if (n_reads == 0) return DTWAX_OK;
+  if (!d_q || !d_off || !d_len || !d_out) return fail(DTWAX_EINVAL, "NULL argument");
+  HIP_TRY(hipSetDevice(c->device));
+  std::vector<int32_t> lens((size_t)n_reads);
+  HIP_TRY(hipMemcpyAsync(lens.data(), d_len, (size_t)n_reads * 4, hipMemcpyDeviceToHost, S_(st)));
+  HIP_TRY(hipStreamSynchronize(S_(st)));
+  dtwax_plan* p = nullptr;
+  int rc = dtwax_plan_create(c, lens.data(), n_reads, opts, &p);
+  if (rc) return rc;
+  rc = dtwax_score_plan(p, d_q, d_off, d_len, d_out, st);
+  if (rc == DTWAX_OK) {
+    hipError_t e = hipStreamSynchronize(S_(st));
+    if (e != hipSuccess) rc = fail(DTWAX_EDEVICE, std::string("score: ") + hipGetErrorString(e));
+  }
+  dtwax_plan_destroy(p);
+  return rc;
+}
+
+// ---- reference-compatible entry point ------------------------------------
+// Geometry of the reference build (src/include/common.hpp:60-98, -DFP16).
+static const int32_t kRefQuery = 2048, kRefSeg = 26, kRefWarp = 32, kRefLen = 48128;
+
+int dtwax_distances(dtwax_ctx* c, const void* ref, const void* query, void* dists, int32_t num_entries,
+                    uint32_t thresh, void* stream, void* device_last_row, int32_t compat_quirks) {
+  (void)thresh;           // unused by the reference kernel as well (DTW.cu:152-153)
+  (void)device_last_row;  // band scratch is internal
+  if (!c || !ref || !query || !dists) return fail(DTWAX_EINVAL, "NULL argument");
+  if (num_entries < 0) return fail(DTWAX_EINVAL, "num_entries < 0");
+  if (compat_quirks) return fail(DTWAX_EINVAL, "compat_quirks=1 is not built into this library yet");
+  if (num_entries == 0) return DTWAX_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t st = S_(stream);
+  const int32_t tile = kRefSeg * kRefWarp;
+  const int32_t n_tiles = kRefLen / tile;  // 57: columns beyond 57*832 are never scored
+  const int32_t n_cols = n_tiles * tile;
+  // reference: untile into a private padded buffer and make it current
+  if (c->n_cols != n_cols || c->n_strands != 2 || !c->compat_ref) {
+    std::vector<float> zeros((size_t)2 * n_cols, 0.0f);
+    int rc = dtwax_reference_upload(c, zeros.data(), 2, n_cols);
+    if (rc) return rc;
+    c->compat_ref = c->ref16_alloc;
+  }
+  HIP_TRY(launch_untile_reference((const uint32_t*)ref, kRefLen, kRefSeg, kRefWarp, n_tiles,
+                                  (uint32_t*)c->ref16_alloc + COL_PADL, st));
+  // queries
+  const int64_t nq = (int64_t)num_entries * kRefQuery;
+  if (c->compat_cap < num_entries) {
+    for (void* p : {(void*)c->compat_q, (void*)c->compat_off, (void*)c->compat_len, (void*)c->compat_res})
+      if (p) (void)hipFree(p);
+    if (hipMalloc(&c->compat_q, (size_t)nq * 4) != hipSuccess ||
+        hipMalloc(&c->compat_off, (size_t)num_entries * 8) != hipSuccess ||
+        hipMalloc(&c->compat_len, (size_t)num_entries * 4) != hipSuccess ||
+        hipMalloc(&c->compat_res, (size_t)num_entries * sizeof(DeviceResult)) != hipSuccess) {
+      c->compat_cap = 0;
+      return fail(DTWAX_ENOMEM, "compat buffers");
+    }
+    std::vector<int64_t> off((size_t)num_entries);
+    std::vector<int32_t> len((size_t)num_entries, kRefQuery);
+    for (int32_t i = 0; i < num_entries; i++) off[(size_t)i] = (int64_t)i * kRefQuery;
+    HIP_TRY(hipMemcpy(c->compat_off, off.data(), off.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->compat_len, len.data(), len.size() * 4, hipMemcpyHostToDevice));
+    c->compat_cap = num_entries;
+  }
+  HIP_TRY(launch_half2_queries_to_float((const uint32_t*)query, nq, c->compat_q, st));
+  if (!c->compat_plan || c->compat_plan_n != num_entries) {
+    if (c->compat_plan) dtwax_plan_destroy(c->compat_plan);
+    c->compat_plan = nullptr;
+    std::vector<int32_t> lens((size_t)num_entries, kRefQuery);
+    dtwax_score_opts o{};
+    o.precision = DTWAX_F16;
+    o.segment = 32;  // 64 lanes x 32 rows = QUERY_LEN: one band, no padding rows
+    o.ref_chunk = n_cols;
+    int rc = dtwax_plan_create(c, lens.data(), num_entries, &o, &c->compat_plan);
+    if (rc) return rc;
+    c->compat_plan_n = num_entries;
+  }
+  int rc = run_plan(c->compat_plan, c->compat_q, c->compat_off, c->compat_len,
+                    reinterpret_cast<dtwax_result*>(c->compat_res), st, nullptr, nullptr, 0, nullptr, 0);
+  if (rc) return rc;
+  HIP_TRY(launch_results_to_half2(c->compat_res, num_entries, (uint32_t*)dists, st));
+  return DTWAX_OK;
+}
+
+// ---- self test ---------------------------------------------------------------
+int dtwax_selftest_lanes(int32_t* out_host) {
+  if (!out_host) return fail(DTWAX_EINVAL, "NULL argument");
+  int32_t* d = nullptr;
+  HIP_TRY(hipMalloc(&d, 128 * 4));
+  hipError_t e = launch_selftest_lanes(d, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(out_host, d, 128 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(DTWAX_EDEVICE, std::string("selftest: ") + hipGetErrorString(e));
+  return DTWAX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// kernels.h -- device-side structures shared by the HIP kernels and the host
+// launcher (internal; the public C ABI is include/dtwax.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dtwax.h"
+
+namespace dtwax {
+
+// Padding (in columns) around every device-resident row: the reference, the
+// band scratch rows and the Read Until state rows.  The sweep reads columns
+// [x - 64, c1 + 253].
+constexpr int COL_PADL = 64;
+constexpr int COL_PADR = 384;
+constexpr int SCRATCH_PADL = COL_PADL;
+
+// One work item: read `read` against reference columns [x, c1), counting the
+// min only over [c0, c1) (x = c0 - (Q-1) clamped at 0: columns [x, c0) are
+// the overlap that makes a chunk exact under the no-deletion recurrence).
+struct WorkItem {
+  int32_t read;
+  int32_t slot;  // Read Until state slot (row_in / row_out), else 0
+  int32_t x;
+  int32_t c0;
+  int32_t c1;
+  int32_t reserved[3];
+};
+
+struct DeviceResult {
+  float score[2];
+  int32_t argmin[2];
+};
+
+struct KParams {
+  const void* ref;  // column 0 of the padded reference (h2 or NSTR floats per column)
+  int32_t n_cols;
+  int32_t n_items;
+  const float* queries;
+  const int64_t* q_off;
+  const int32_t* q_len;
+  const WorkItem* items;
+  int32_t* counter;
+  unsigned long long* keys;  // 2 per read: (ordered score bits << 32) | column
+  void* scratch;             // per workgroup band rows
+  int64_t scratch_stride;    // columns per workgroup (includes the pads)
+  const void* row_in;        // Read Until: column 0 of slot 0's input row
+  void* row_out;             // Read Until: column 0 of slot 0's output row
+  int64_t row_stride;        // columns per slot
+  float renorm_k;            // F16_RENORM per-lane drift constant (S rows' worth)
+};
+
+hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool inj,
+                       hipStream_t st);
+hipError_t launch_reset(int32_t* counter, unsigned long long* keys, int64_t n_keys, hipStream_t st);
+hipError_t launch_finalize(const unsigned long long* keys, int32_t n_reads, int32_t n_strands,
+                           DeviceResult* out, hipStream_t st);
+hipError_t launch_selftest_lanes(int32_t* d_out, hipStream_t st);
+int occupancy_f16(int precision, int S, bool inj);
+int occupancy_f32(int S, int nstr, bool inj);
+
+// normalisation (normalize_kernels.hip)
+hipError_t launch_normalize(const float* raw, const int64_t* off, const int32_t* len, int32_t n_reads,
+                            int mode, double eps, float* out, double* stats, hipStream_t st);
+
+// reference-compat helpers (compat_kernels.hip)
+hipError_t launch_untile_reference(const uint32_t* tiled, int32_t ref_len, int32_t seg, int32_t warp,
+                                   int32_t n_tiles, uint32_t* untiled_h2, hipStream_t st);
+hipError_t launch_half2_queries_to_float(const uint32_t* q_h2, int64_t n, float* out, hipStream_t st);
+hipError_t launch_results_to_half2(const DeviceResult* res, int32_t n, uint32_t* dists, hipStream_t st);
+
+}  // namespace dtwax

@@ -1,0 +1,644 @@
+// sdtw_kernels.hip -- gfx950 subsequence-DTW kernels.
+//
+// Recurrence (reference build -DNO_REF_DEL, src/include/DTW.cu:37-39):
+//   D[r][c] = (s_c - q_r)^2 + min(D[r-1][c], D[r-1][c-1])
+//   D[-1][c] = top row (0 for a free start), D[r][-1] = +inf
+//   score = min_c D[Q-1][c]
+//
+// There is no horizontal move, so a row depends only on the row above.  One
+// wavefront owns one work item (read, reference column range).  The query is
+// register resident: lane L holds rows [band0 + L*S, band0 + (L+1)*S) of the
+// current band (64*S rows per band), and the wave sweeps reference columns
+// with a one-step skew per lane (lane L is at column x + t - L at step t).
+// Per step a lane updates its S cells of one column:
+//   t = fma(s, a_i, -q_i); D_i = t*t + min(D_{i-1}(new), D_{i-1}(old))
+// where a_i = 1 for real rows and 0 for the virtual rows that pad the first
+// band (their cost is 0, so they carry the free-start row through).  Cells
+// pass between lanes with one DPP wave_shr:1 per step (the lane above's last
+// cell becomes this lane's top input; last step's top input is the diagonal).
+// The last row of the band (lane 63, register S-1) is gathered over 64 steps
+// into a history register with one DPP wave_shl:1 per step and then flushed:
+// to the running min/argmin (final band) or to a row buffer (next band, or
+// the Read Until state).
+//
+// Reference columns and the top-boundary row are streamed HBM -> LDS ring
+// (256 slots + a 64-slot mirror so a lane reads 64 consecutive slots without
+// wrapping) two blocks ahead with coalesced loads; each step reads its column
+// from LDS with an immediate offset.  No MFMA: this is a min-plus recurrence.
+//
+// Compiled with -ffp-contract=off (the binary16 form must round t*t and the
+// sum separately, as the reference does) and -fno-honor-nans (no NaN ever
+// reaches the min, and without it every min is preceded by a canonicalise).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace dtwax {
+
+typedef _Float16 half_t;
+typedef half_t h2 __attribute__((ext_vector_type(2)));
+
+#define DPP_WAVE_SHL1 0x130
+#define DPP_WAVE_SHR1 0x138
+
+template <class T>
+__device__ __forceinline__ T dpp_shr1(T old, T src) {
+  static_assert(sizeof(T) == 4, "32-bit DPP");
+  int r = __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src),
+                                      DPP_WAVE_SHR1, 0xf, 0xf, false);
+  return __builtin_bit_cast(T, r);
+}
+template <class T>
+__device__ __forceinline__ T dpp_shl1(T old, T src) {
+  static_assert(sizeof(T) == 4, "32-bit DPP");
+  int r = __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src),
+                                      DPP_WAVE_SHL1, 0xf, 0xf, false);
+  return __builtin_bit_cast(T, r);
+}
+
+__device__ __forceinline__ h2 hsplat(half_t v) { return (h2){v, v}; }
+__device__ __forceinline__ h2 hmin2(h2 a, h2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ h2 hfma2(h2 a, h2 b, h2 c) { return __builtin_elementwise_fma(a, b, c); }
+
+// Order-preserving map of a float to uint32 (for the 64-bit min keys).
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t make_key(float v, int32_t col) {
+  return ((uint64_t)ordered_bits(v) << 32) | (uint32_t)col;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+constexpr int RING = 256;        // ring slots
+constexpr int RING_ALLOC = 320;  // + mirror of slots [0, 64)
+
+// ------------------------------------------------------------------------
+// binary16 forms (DTWAX_F16, DTWAX_F16_FUSED, DTWAX_F16_RENORM)
+// ------------------------------------------------------------------------
+template <int S, int MODE>
+struct F16Band {
+  h2 Qn[S];  // {-q, -q}
+  h2 A[S];   // {1, 1} real row, {0, 0} virtual row
+};
+
+// One step: Do = column c-1 values, Dn = column c values.
+template <int S, int MODE>
+__device__ __forceinline__ void f16_cells(const F16Band<S, MODE>& B, h2 s, h2 tin, h2 dg,
+                                          const h2 (&Do)[S], h2 (&Dn)[S]) {
+  h2 tp = tin, dd = dg;
+#pragma unroll
+  for (int i = 0; i < S; i++) {
+    h2 t = hfma2(s, B.A[i], B.Qn[i]);  // == rn(s - q) for a real row (a = 1)
+    h2 m = hmin2(tp, dd);
+    h2 v;
+    if constexpr (MODE == DTWAX_F16_FUSED) {
+      v = hfma2(t, t, m);
+    } else {
+      h2 c = t * t;
+      v = c + m;
+    }
+    Dn[i] = v;
+    dd = Do[i];
+    tp = v;
+  }
+}
+
+struct SweepIO {
+  const h2* ref;   // column 0 of the padded reference
+  const h2* trow;  // top-boundary row (absolute columns) or null for a free start
+  h2* orow;        // row output (absolute columns) or null
+  int32_t x, c0, c1;
+  int32_t inj_lane;  // lane that takes the top row (first real lane)
+  bool final_band;
+};
+
+// Sweep one band across columns [x, c1) (+ ramp).  Accumulates the min/argmin
+// of the band's last row over [c0, c1) into best/pos when final_band.
+template <int S, int MODE, bool INJ>
+__device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const SweepIO& io, h2 K,
+                                          h2* __restrict__ ring_s, h2* __restrict__ ring_t,
+                                          int lane, h2& best, int32_t& posf, int32_t& posr,
+                                          h2& rowmin) {
+  const h2 INF = hsplat((half_t)__builtin_inff());
+  const h2 ZERO = hsplat((half_t)0.0f);
+  const int32_t x = io.x;
+  const int32_t nsteps = ((io.c1 - x + 63) + 63) & ~63;
+  const h2* __restrict__ ref = io.ref;
+  const h2* __restrict__ trow = io.trow;
+
+  // Prologue: columns [x-64, x+128) into slots [0, 192) (+ mirror of [0, 64)).
+  {
+    h2 a = ref[x - 64 + lane], b = ref[x + lane], c = ref[x + 64 + lane];
+    ring_s[lane] = a;
+    ring_s[RING + lane] = a;
+    ring_s[64 + lane] = b;
+    ring_s[128 + lane] = c;
+    h2 ta = ZERO, tb = ZERO, tc = ZERO;
+    if (trow) { ta = trow[x - 64 + lane]; tb = trow[x + lane]; tc = trow[x + 64 + lane]; }
+    ring_t[lane] = ta;
+    ring_t[RING + lane] = ta;
+    ring_t[64 + lane] = tb;
+    ring_t[128 + lane] = tc;
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+
+  h2 D0[S], D1[S];
+#pragma unroll
+  for (int i = 0; i < S; i++) D0[i] = INF;
+  h2 tin = INF, dg = INF, hist = INF;
+  if (lane == 0) tin = ring_t[64] - K;  // top row at column x for lane 0
+  if (INJ && lane == io.inj_lane) tin = ring_t[64 - lane] - K;
+
+  for (int32_t T = 0; T < nsteps; T += 64) {
+    // Stream block T+128 .. T+191 (written to the ring after this block).
+    h2 pf_s = ref[x + T + 128 + lane];
+    h2 pf_t = trow ? trow[x + T + 128 + lane] : ZERO;
+    const int base = (64 - lane + T) & (RING - 1);
+#pragma unroll 1
+    for (int g = 0; g < 64; g += 8) {
+      const h2* rs = ring_s + base + g;
+      const h2* rt = ring_t + base + g;
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        {
+          h2 s = rs[u];
+          f16_cells<S, MODE>(B, s, tin, dg, D0, D1);
+          h2 tf = rt[u + 1];
+          dg = tin;
+          tin = dpp_shr1(tf, D1[S - 1]);
+          if (INJ) tin = (lane == io.inj_lane) ? tf : tin;
+          if (MODE == DTWAX_F16_RENORM) tin = tin - K;
+          hist = dpp_shl1(D1[S - 1], hist);
+        }
+        {
+          h2 s = rs[u + 1];
+          f16_cells<S, MODE>(B, s, tin, dg, D1, D0);
+          h2 tf = rt[u + 2];
+          dg = tin;
+          tin = dpp_shr1(tf, D0[S - 1]);
+          if (INJ) tin = (lane == io.inj_lane) ? tf : tin;
+          if (MODE == DTWAX_F16_RENORM) tin = tin - K;
+          hist = dpp_shl1(D0[S - 1], hist);
+        }
+      }
+    }
+    // hist lane j = last-row value of column x + T + j - 63
+    const int32_t col = x + T + lane - 63;
+    const bool in_item = (col >= x) && (col < io.c1);
+    if (io.final_band) {
+      if (col >= io.c0 && col < io.c1) {
+        if (hist.x < best.x) { best.x = hist.x; posf = col; }
+        if (hist.y < best.y) { best.y = hist.y; posr = col; }
+      }
+    }
+    if (MODE == DTWAX_F16_RENORM && !io.final_band && in_item) rowmin = hmin2(rowmin, hist);
+    if (io.orow && in_item) io.orow[col] = hist;
+    // Ring refill for columns x + T + 128 + lane.
+    __builtin_amdgcn_wave_barrier();
+    const int slot = (T + 192 + lane) & (RING - 1);
+    ring_s[slot] = pf_s;
+    ring_t[slot] = pf_t;
+    if (slot < 64) { ring_s[RING + slot] = pf_s; ring_t[RING + slot] = pf_t; }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+__device__ __forceinline__ half_t clamp_half(float v) { return (half_t)v; }
+
+template <int S, int MODE, bool INJ>
+__device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* ring_s, h2* ring_t,
+                         h2* scratch) {
+  const int32_t read = it.read;
+  const int32_t Q = p.q_len[read];
+  const float* __restrict__ qp = p.queries + p.q_off[read];
+  constexpr int BR = 64 * S;
+  const int32_t nb = (Q + BR - 1) / BR;
+  const int32_t P = nb * BR - Q;  // virtual rows in front of band 0
+  const h2 K = (MODE == DTWAX_F16_RENORM) ? hsplat((half_t)p.renorm_k) : hsplat((half_t)0.0f);
+
+  const h2* trow_ext = nullptr;
+  h2* orow_ext = nullptr;
+  if (p.row_in) trow_ext = (const h2*)p.row_in + (int64_t)it.slot * p.row_stride;
+  if (p.row_out) orow_ext = (h2*)p.row_out + (int64_t)it.slot * p.row_stride;
+
+  h2 best = hsplat((half_t)__builtin_inff());
+  int32_t posf = -1, posr = -1;
+  // F16_RENORM: every lane subtracts Kl from its top input, a per-row
+  // constant (the same for all columns), so the last row ends up lowered by
+  // the sum of the subtractions on its path; that sum is added back here in
+  // binary32.  Lane 0 of a band after the first also subtracts the exact
+  // minimum of the row the previous band left, per strand.
+  float off_f = 0.0f, off_r = 0.0f;
+  h2 prev_min = hsplat((half_t)0.0f);
+
+  for (int32_t b = 0; b < nb; b++) {
+    F16Band<S, MODE> B;
+    const int32_t r0 = b * BR - P + lane * S;
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+      const int32_t r = r0 + i;
+      if (r >= 0) {
+        half_t q = (half_t)qp[r];
+        B.Qn[i] = hsplat(-q);
+        B.A[i] = hsplat((half_t)1.0f);
+      } else {
+        B.Qn[i] = hsplat((half_t)0.0f);
+        B.A[i] = hsplat((half_t)0.0f);
+      }
+    }
+    SweepIO io;
+    io.ref = (const h2*)p.ref;
+    // the band scratch row is indexed by absolute column; it holds [x - 64, c1 + 384)
+    h2* scr = scratch - it.x;
+    io.trow = (b == 0) ? trow_ext : scr;
+    io.final_band = (b == nb - 1);
+    io.orow = io.final_band ? orow_ext : scr;
+    io.x = it.x;
+    io.c0 = it.c0;
+    io.c1 = it.c1;
+    io.inj_lane = (b == 0) ? (P / S) : 0;
+    h2 rowmin = hsplat((half_t)__builtin_inff());
+    h2 Kl = K;
+    if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = K + prev_min;
+    f16_sweep<S, MODE, INJ>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
+    if (MODE == DTWAX_F16_RENORM) {
+      // lanes whose subtraction lies on the path to the last row
+      const int32_t first = INJ ? io.inj_lane : 0;
+      const h2 k0 = __builtin_bit_cast(h2, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Kl), first));
+      off_f += (float)(63 - first) * (float)K.x + (float)k0.x;
+      off_r += (float)(63 - first) * (float)K.y + (float)k0.y;
+      if (!io.final_band) {
+        h2 m = rowmin;
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+          h2 o = __builtin_bit_cast(h2, __shfl_xor(__builtin_bit_cast(int, m), sh, 64));
+          m = hmin2(m, o);
+        }
+        prev_min = m;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __threadfence_block();
+  }
+
+  // Combine: per strand min value (as float + offset), lowest column on ties.
+  uint64_t kf = (posf >= 0) ? make_key((float)best.x + off_f, posf) : ~0ull;
+  uint64_t kr = (posr >= 0) ? make_key((float)best.y + off_r, posr) : ~0ull;
+  kf = wave_min_u64(kf);
+  kr = wave_min_u64(kr);
+  if (lane == 0) {
+    if (kf != ~0ull) atomicMin((unsigned long long*)&p.keys[2 * (int64_t)read], (unsigned long long)kf);
+    if (kr != ~0ull) atomicMin((unsigned long long*)&p.keys[2 * (int64_t)read + 1], (unsigned long long)kr);
+  }
+}
+
+template <int S, int MODE, bool INJ>
+__global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
+  __shared__ h2 ring_s[RING_ALLOC];
+  __shared__ h2 ring_t[RING_ALLOC];
+  const int lane = threadIdx.x;
+  h2* scratch = (h2*)p.scratch + (int64_t)blockIdx.x * p.scratch_stride + SCRATCH_PADL;
+  for (;;) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(p.counter, 1);
+    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0, 64));
+    if (idx >= p.n_items) break;
+    WorkItem it = p.items[idx];
+    f16_item<S, MODE, INJ>(p, it, lane, ring_s, ring_t, scratch);
+  }
+}
+
+// ------------------------------------------------------------------------
+// binary32 form (DTWAX_F32), NSTR strands (1: single sequence, 2: fwd/rev)
+// ------------------------------------------------------------------------
+template <int S>
+__device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)[S], float s,
+                                          float tin, float dg, const float (&Do)[S],
+                                          float (&Dn)[S]) {
+  float tp = tin, dd = dg;
+#pragma unroll
+  for (int i = 0; i < S; i++) {
+    float t = __builtin_fmaf(s, A[i], Qn[i]);  // == rn(s - q) for a real row
+    float c = t * t;
+    float m = __builtin_fminf(tp, dd);
+    float v = c + m;
+    Dn[i] = v;
+    dd = Do[i];
+    tp = v;
+  }
+}
+
+template <int S, int NSTR, bool INJ>
+__device__ __forceinline__ void f32_half_step(const float (&A)[S], const float (&Qn)[S],
+                                              const float* rs, const float* rt, int u,
+                                              float (&tin)[NSTR], float (&dg)[NSTR],
+                                              float (&hist)[NSTR], float (&Do)[NSTR][S],
+                                              float (&Dn)[NSTR][S], int lane, int inj) {
+#pragma unroll
+  for (int st = 0; st < NSTR; st++) {
+    const float s = rs[u * NSTR + st];
+    f32_cells<S>(A, Qn, s, tin[st], dg[st], Do[st], Dn[st]);
+    const float tf = rt[(u + 1) * NSTR + st];
+    dg[st] = tin[st];
+    float ti = dpp_shr1(tf, Dn[st][S - 1]);
+    if (INJ) ti = (lane == inj) ? tf : ti;
+    tin[st] = ti;
+    hist[st] = dpp_shl1(Dn[st][S - 1], hist[st]);
+  }
+}
+
+template <int S, int NSTR, bool INJ>
+__device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* ring_s,
+                         float* ring_t, float* scratch) {
+  const int32_t read = it.read;
+  const int32_t Q = p.q_len[read];
+  const float* __restrict__ qp = p.queries + p.q_off[read];
+  constexpr int BR = 64 * S;
+  const int32_t nb = (Q + BR - 1) / BR;
+  const int32_t P = nb * BR - Q;
+  const float INF = __builtin_inff();
+  const float* __restrict__ ref = (const float*)p.ref;  // NSTR floats per column
+  const float* trow_ext = nullptr;
+  float* orow_ext = nullptr;
+  if (p.row_in) trow_ext = (const float*)p.row_in + (int64_t)it.slot * p.row_stride * NSTR;
+  if (p.row_out) orow_ext = (float*)p.row_out + (int64_t)it.slot * p.row_stride * NSTR;
+
+  float best[NSTR];
+  int32_t pos[NSTR];
+#pragma unroll
+  for (int k = 0; k < NSTR; k++) { best[k] = INF; pos[k] = -1; }
+
+  for (int32_t b = 0; b < nb; b++) {
+    float Qn[S], A[S];
+    const int32_t r0 = b * BR - P + lane * S;
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+      const int32_t r = r0 + i;
+      Qn[i] = (r >= 0) ? -qp[r] : 0.0f;
+      A[i] = (r >= 0) ? 1.0f : 0.0f;
+    }
+    float* scr = scratch - (int64_t)it.x * NSTR;  // absolute-column view of the band scratch
+    const float* trow = (b == 0) ? trow_ext : scr;
+    const bool final_band = (b == nb - 1);
+    float* orow = final_band ? orow_ext : scr;
+    const int32_t x = it.x, c1 = it.c1, c0 = it.c0;
+    const int32_t inj = (b == 0) ? (P / S) : 0;
+    const int32_t nsteps = ((c1 - x + 63) + 63) & ~63;
+
+    for (int k = 0; k < 3; k++) {
+      const int32_t c = x - 64 + 64 * k + lane;
+#pragma unroll
+      for (int st = 0; st < NSTR; st++) {
+        float v = ref[(int64_t)c * NSTR + st];
+        float tv = trow ? trow[(int64_t)c * NSTR + st] : 0.0f;
+        ring_s[(64 * k + lane) * NSTR + st] = v;
+        ring_t[(64 * k + lane) * NSTR + st] = tv;
+        if (k == 0) {
+          ring_s[(RING + lane) * NSTR + st] = v;
+          ring_t[(RING + lane) * NSTR + st] = tv;
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+
+    float D0[NSTR][S], D1[NSTR][S];
+    float tin[NSTR], dg[NSTR], hist[NSTR];
+#pragma unroll
+    for (int st = 0; st < NSTR; st++) {
+#pragma unroll
+      for (int i = 0; i < S; i++) D0[st][i] = INF;
+      tin[st] = INF;
+      dg[st] = INF;
+      hist[st] = INF;
+      if (lane == 0) tin[st] = ring_t[64 * NSTR + st];
+      if (INJ && lane == inj) tin[st] = ring_t[(64 - lane) * NSTR + st];
+    }
+
+    for (int32_t T = 0; T < nsteps; T += 64) {
+      float pf_s[NSTR], pf_t[NSTR];
+#pragma unroll
+      for (int st = 0; st < NSTR; st++) {
+        pf_s[st] = ref[(int64_t)(x + T + 128 + lane) * NSTR + st];
+        pf_t[st] = trow ? trow[(int64_t)(x + T + 128 + lane) * NSTR + st] : 0.0f;
+      }
+      const int base = (64 - lane + T) & (RING - 1);
+#pragma unroll 1
+      for (int g = 0; g < 64; g += 8) {
+        const float* rs = ring_s + (base + g) * NSTR;
+        const float* rt = ring_t + (base + g) * NSTR;
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          f32_half_step<S, NSTR, INJ>(A, Qn, rs, rt, u, tin, dg, hist, D0, D1, lane, inj);
+          f32_half_step<S, NSTR, INJ>(A, Qn, rs, rt, u + 1, tin, dg, hist, D1, D0, lane, inj);
+        }
+      }
+      const int32_t col = x + T + lane - 63;
+      const bool in_item = (col >= x) && (col < c1);
+#pragma unroll
+      for (int st = 0; st < NSTR; st++) {
+        if (final_band && col >= c0 && col < c1 && hist[st] < best[st]) {
+          best[st] = hist[st];
+          pos[st] = col;
+        }
+        if (orow && in_item) orow[(int64_t)col * NSTR + st] = hist[st];
+      }
+      __builtin_amdgcn_wave_barrier();
+      const int slot = (T + 192 + lane) & (RING - 1);
+#pragma unroll
+      for (int st = 0; st < NSTR; st++) {
+        ring_s[slot * NSTR + st] = pf_s[st];
+        ring_t[slot * NSTR + st] = pf_t[st];
+        if (slot < 64) {
+          ring_s[(RING + slot) * NSTR + st] = pf_s[st];
+          ring_t[(RING + slot) * NSTR + st] = pf_t[st];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __threadfence_block();
+  }
+#pragma unroll
+  for (int st = 0; st < NSTR; st++) {
+    uint64_t k = (pos[st] >= 0) ? make_key(best[st], pos[st]) : ~0ull;
+    k = wave_min_u64(k);
+    if (lane == 0 && k != ~0ull)
+      atomicMin((unsigned long long*)&p.keys[2 * (int64_t)read + st], (unsigned long long)k);
+  }
+}
+
+template <int S, int NSTR, bool INJ>
+__global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
+  __shared__ float ring_s[RING_ALLOC * NSTR];
+  __shared__ float ring_t[RING_ALLOC * NSTR];
+  const int lane = threadIdx.x;
+  float* scratch = (float*)p.scratch + ((int64_t)blockIdx.x * p.scratch_stride + SCRATCH_PADL) * NSTR;
+  for (;;) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(p.counter, 1);
+    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0, 64));
+    if (idx >= p.n_items) break;
+    WorkItem it = p.items[idx];
+    f32_item<S, NSTR, INJ>(p, it, lane, ring_s, ring_t, scratch);
+  }
+}
+
+// ------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------
+__global__ void reset_score_kernel(int32_t* counter, unsigned long long* keys, int64_t n_keys) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *counter = 0;
+  for (; i < n_keys; i += (int64_t)gridDim.x * blockDim.x) keys[i] = ~0ull;
+}
+
+__global__ void finalize_kernel(const unsigned long long* keys, int32_t n_reads, int32_t n_strands,
+                                DeviceResult* out) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_reads) return;
+  DeviceResult r;
+  for (int k = 0; k < 2; k++) {
+    unsigned long long key = (k < n_strands) ? keys[2 * (int64_t)i + k] : ~0ull;
+    if (key == ~0ull) {
+      r.score[k] = __builtin_inff();
+      r.argmin[k] = -1;
+    } else {
+      uint32_t hi = (uint32_t)(key >> 32);
+      uint32_t u = (hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi;
+      r.score[k] = __builtin_bit_cast(float, u);
+      r.argmin[k] = (int32_t)(key & 0xffffffffu);
+    }
+  }
+  out[i] = r;
+}
+
+__global__ void selftest_lanes_kernel(int32_t* out) {
+  int lane = threadIdx.x;
+  out[lane] = dpp_shr1(-1, lane);
+  out[64 + lane] = dpp_shl1(-1, lane);
+}
+
+// ------------------------------------------------------------------------
+// launch table
+// ------------------------------------------------------------------------
+template <int S, int MODE>
+static hipError_t launch_f16(const KParams& p, int grid, bool inj, hipStream_t st) {
+  if (inj)
+    hipLaunchKernelGGL((sdtw_f16_kernel<S, MODE, true>), dim3(grid), dim3(64), 0, st, p);
+  else
+    hipLaunchKernelGGL((sdtw_f16_kernel<S, MODE, false>), dim3(grid), dim3(64), 0, st, p);
+  return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_f32(const KParams& p, int grid, int nstr, bool inj, hipStream_t st) {
+  if (nstr == 1) {
+    if (inj) hipLaunchKernelGGL((sdtw_f32_kernel<S, 1, true>), dim3(grid), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL((sdtw_f32_kernel<S, 1, false>), dim3(grid), dim3(64), 0, st, p);
+  } else {
+    if (inj) hipLaunchKernelGGL((sdtw_f32_kernel<S, 2, true>), dim3(grid), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL((sdtw_f32_kernel<S, 2, false>), dim3(grid), dim3(64), 0, st, p);
+  }
+  return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_f16_s(const KParams& p, int S, int grid, bool inj, hipStream_t st) {
+  switch (S) {
+    case 8: return launch_f16<8, MODE>(p, grid, inj, st);
+    case 16: return launch_f16<16, MODE>(p, grid, inj, st);
+    case 25: return launch_f16<25, MODE>(p, grid, inj, st);
+    case 32: return launch_f16<32, MODE>(p, grid, inj, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool inj,
+                       hipStream_t st) {
+  switch (precision) {
+    case DTWAX_F16: return launch_f16_s<DTWAX_F16>(p, S, grid, inj, st);
+    case DTWAX_F16_FUSED: return launch_f16_s<DTWAX_F16_FUSED>(p, S, grid, inj, st);
+    case DTWAX_F16_RENORM: return launch_f16_s<DTWAX_F16_RENORM>(p, S, grid, inj, st);
+    case DTWAX_F32:
+      switch (S) {
+        case 8: return launch_f32<8>(p, grid, nstr, inj, st);
+        case 16: return launch_f32<16>(p, grid, nstr, inj, st);
+        case 25: return launch_f32<25>(p, grid, nstr, inj, st);
+        case 32: return launch_f32<32>(p, grid, nstr, inj, st);
+      }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_reset(int32_t* counter, unsigned long long* keys, int64_t n_keys, hipStream_t st) {
+  int64_t blocks = (n_keys + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(reset_score_kernel, dim3((unsigned)blocks), dim3(256), 0, st, counter, keys, n_keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const unsigned long long* keys, int32_t n_reads, int32_t n_strands,
+                           DeviceResult* out, hipStream_t st) {
+  int blocks = (n_reads + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(finalize_kernel, dim3(blocks), dim3(256), 0, st, keys, n_reads, n_strands, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_selftest_lanes(int32_t* d_out, hipStream_t st) {
+  hipLaunchKernelGGL(selftest_lanes_kernel, dim3(1), dim3(64), 0, st, d_out);
+  return hipGetLastError();
+}
+
+int occupancy_f16(int precision, int S, bool inj) {
+  int n = 0;
+#define OCC(MODE, SS)                                                                          \
+  if (inj) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<SS, MODE, true>, 64, 0); \
+  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<SS, MODE, false>, 64, 0);
+#define OCC_S(MODE)            \
+  switch (S) {                 \
+    case 8: OCC(MODE, 8) break;   \
+    case 16: OCC(MODE, 16) break; \
+    case 25: OCC(MODE, 25) break; \
+    case 32: OCC(MODE, 32) break; \
+  }
+  switch (precision) {
+    case DTWAX_F16: OCC_S(DTWAX_F16) break;
+    case DTWAX_F16_FUSED: OCC_S(DTWAX_F16_FUSED) break;
+    case DTWAX_F16_RENORM: OCC_S(DTWAX_F16_RENORM) break;
+  }
+#undef OCC_S
+#undef OCC
+  return n;
+}
+
+int occupancy_f32(int S, int nstr, bool inj) {
+  int n = 0;
+#define OCC(SS, NS)                                                                            \
+  if (inj) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<SS, NS, true>, 64, 0); \
+  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<SS, NS, false>, 64, 0);
+  switch (S) {
+    case 8: if (nstr == 1) { OCC(8, 1) } else { OCC(8, 2) } break;
+    case 16: if (nstr == 1) { OCC(16, 1) } else { OCC(16, 2) } break;
+    case 25: if (nstr == 1) { OCC(25, 1) } else { OCC(25, 2) } break;
+    case 32: if (nstr == 1) { OCC(32, 1) } else { OCC(32, 2) } break;
+  }
+#undef OCC
+  return n;
+}
+
+}  // namespace dtwax

@@ -1,0 +1,188 @@
+/*
+ * dtwax.h -- C ABI of the MI355X (gfx950) sDTW read classifier.
+ *
+ * Plain C types and pointers only.  Every entry point returns a dtwax_status
+ * (0 = success, negative = error; dtwax_last_error() gives the message).
+ * Device pointers are addresses returned by dtwax_malloc (or any HIP device
+ * allocation in the same process); `stream` is a hipStream_t passed as
+ * void* (NULL = the default stream).  Work is asynchronous on `stream`
+ * unless a function says otherwise.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * DTWax checkout):
+ *   dtwax_distances        FullDTW::distances            src/include/DTW.hpp:285-295
+ *   dtwax_score            the batch loop + distances     src/main.cu:211-274
+ *   dtwax_normalize        normalizer::normalize          src/include/normalizer.cu:71-114
+ *   dtwax_reference_*      load_reference + tile upload   src/include/load_reference.hpp:33-108,
+ *                                                         src/main.cu:78-121
+ *   dtwax_squiggle_build   load_reference::load_ref_coeffs src/include/load_reference.hpp:33-108
+ *   dtwax_ru_*             the Read Until loop            src/ru_main.cu:172-326
+ */
+#ifndef DTWAX_H
+#define DTWAX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  DTWAX_OK = 0,
+  DTWAX_EINVAL = -1,   /* bad argument */
+  DTWAX_ENOMEM = -2,   /* device or host allocation failed */
+  DTWAX_EDEVICE = -3,  /* HIP runtime error */
+  DTWAX_ENOTREADY = -4,/* call order error (e.g. no reference uploaded) */
+  DTWAX_EIO = -5       /* file could not be read or parsed */
+} dtwax_status;
+
+/* Arithmetic of the DP cell D = (s - q)^2 + min(top, diag). */
+typedef enum {
+  /* binary16 per op, round-to-nearest-even: t = s-q, t2 = t*t, D = t2+min.
+   * The reference's half2 arithmetic (DTW.cu:37-39, common.hpp:32-45). */
+  DTWAX_F16 = 0,
+  /* binary16 with one rounding of t*t + min (fused multiply-add). */
+  DTWAX_F16_FUSED = 1,
+  /* binary16 per op, with per-row drift subtraction at every lane boundary
+   * and exact row-minimum subtraction between bands; the offsets are added
+   * back in binary32. */
+  DTWAX_F16_RENORM = 2,
+  /* binary32 per op (the reference's FP32 build, common.hpp:47-57). */
+  DTWAX_F32 = 3
+} dtwax_precision;
+
+/* Query normalisation. */
+typedef enum {
+  DTWAX_NORM_NONE = 0,     /* copy */
+  DTWAX_NORM_MEANSTD = 1,  /* (x - mean)/sqrt(var_biased + eps) -- normalizer.cu:86-100 */
+  DTWAX_NORM_MEDMAD = 2    /* (x - median)/(1.4826 * MAD) */
+} dtwax_norm_mode;
+
+typedef struct dtwax_ctx dtwax_ctx;
+
+/* Per read and strand (0 = fwd, 1 = rev) result. */
+typedef struct {
+  float score[2];    /* min over the last DP row (strand 1 unused for 1-strand references) */
+  int32_t argmin[2]; /* reference column of that min, lowest on ties; -1 if none */
+} dtwax_result;
+
+typedef struct {
+  int32_t precision;     /* dtwax_precision */
+  int32_t segment;       /* query rows per lane (8, 16, 25 or 32); 0 = automatic */
+  int32_t ref_chunk;     /* reference columns per work item; 0 = automatic (whole strand when
+                            it fits); chunks overlap by (query length - 1) columns, exact for
+                            the no-deletion recurrence */
+  float renorm_drift;    /* DTWAX_F16_RENORM: per-row drift subtracted at lane boundaries;
+                            <= 0 = default 0.1 */
+  int32_t compat_quirks; /* 1: reproduce the reference kernel's band-seam and tile-seam
+                            deviations (DTW.cu:179-180, 315, 385-388, 442); needs the
+                            reference geometry (see dtwax_distances) */
+  int32_t reserved[3];
+} dtwax_score_opts;
+
+/* ---- context ----------------------------------------------------------- */
+const char* dtwax_last_error(void);
+const char* dtwax_version(void);
+/* Create a context on HIP device `device`. */
+int dtwax_create(int device, dtwax_ctx** out);
+void dtwax_destroy(dtwax_ctx* ctx);
+/* Device properties used for the roofline: compute units, peak engine clock. */
+int dtwax_device_info(dtwax_ctx* ctx, int32_t* n_cu, int32_t* clock_khz, char* name,
+                      int32_t name_len);
+
+/* ---- device memory / streams / events (thin HIP wrappers for callers
+ *      without a HIP binding of their own) ------------------------------- */
+int dtwax_malloc(void** dptr, size_t bytes);
+int dtwax_free(void* dptr);
+int dtwax_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int dtwax_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int dtwax_stream_create(void** stream);
+int dtwax_stream_destroy(void* stream);
+int dtwax_stream_sync(void* stream);
+int dtwax_device_sync(void);
+int dtwax_event_create(void** ev);
+int dtwax_event_destroy(void* ev);
+int dtwax_event_record(void* ev, void* stream);
+int dtwax_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---- reference ---------------------------------------------------------
+ * Upload z-scored reference levels: `levels` holds n_strands rows of n_cols
+ * floats (host memory; strand 0 = fwd, 1 = rev).  Replaces any previous
+ * reference.  Synchronous. */
+int dtwax_reference_upload(dtwax_ctx* ctx, const float* levels, int32_t n_strands,
+                           int64_t n_cols);
+/* Build the z-scored squiggle of both strands from a sequence and a k-mer
+ * model (load_reference.hpp:33-108 arithmetic: float sequential sums over the
+ * first ref_len k-mers, sd = sqrt(E[x^2]-mu^2), (level-mu)/sd in float).
+ * seq: ACGT text (length seq_len); model_means: 4^k levels indexed by the
+ * 2-bit k-mer code (A=0, C=1, G=2, T=3, first base most significant);
+ * ref_len: k-mers used (<= seq_len-k+1 for all, or the reference's fixed
+ * REF_LEN); out: 2*ref_len floats (fwd then reverse complement).  Host-side. */
+int dtwax_squiggle_build(const char* seq, int64_t seq_len, int32_t k,
+                         const float* model_means, int64_t ref_len, float* out,
+                         int64_t* n_skipped);
+
+/* ---- normalisation -----------------------------------------------------
+ * Normalise n_reads device-resident reads (float samples; read i starts at
+ * d_offsets[i] and has d_lens[i] samples) into d_out (same layout), with
+ * FP64 statistics on the device.  d_stats (optional, 2*n_reads doubles):
+ * (centre, scale) per read. */
+int dtwax_normalize(dtwax_ctx* ctx, const float* d_raw, const int64_t* d_offsets,
+                    const int32_t* d_lens, int32_t n_reads, int32_t mode, double eps,
+                    float* d_out, double* d_stats, void* stream);
+
+/* ---- batch scoring -----------------------------------------------------
+ * Score n_reads normalised device-resident queries (layout as above) against
+ * the uploaded reference; d_out receives one dtwax_result per read.  opts may
+ * be NULL (F16, automatic tiling).  Reads of any length >= 1 may be mixed in
+ * one call; work is balanced by a persistent kernel over (read, chunk) items
+ * ordered by decreasing cost.  Synchronous (copies the lengths to the host to
+ * plan, runs, and waits); use a plan for asynchronous repeated scoring. */
+int dtwax_score(dtwax_ctx* ctx, const float* d_queries, const int64_t* d_offsets,
+                const int32_t* d_lens, int32_t n_reads, const dtwax_score_opts* opts,
+                dtwax_result* d_out, void* stream);
+
+/* Plans: the work decomposition for a fixed set of read lengths, built once
+ * and reused (dtwax_score builds and frees one per call).  h_lens are host
+ * lengths.  dtwax_score_plan is fully asynchronous on `stream`; calls that
+ * share a plan must not overlap (the plan owns the work counter and keys). */
+typedef struct dtwax_plan dtwax_plan;
+int dtwax_plan_create(dtwax_ctx* ctx, const int32_t* h_lens, int32_t n_reads,
+                      const dtwax_score_opts* opts, dtwax_plan** out);
+void dtwax_plan_destroy(dtwax_plan* plan);
+/* cells = sum over reads of len * counted columns * strands (the GCUPS
+ * numerator); computed = cells actually evaluated (padding rows, chunk
+ * overlap and the 63-step wavefront ramp included). */
+int dtwax_plan_info(const dtwax_plan* plan, int64_t* cells, int64_t* computed,
+                    int32_t* n_items, int32_t* segment, int32_t* ref_chunk, int32_t* grid);
+int dtwax_score_plan(dtwax_plan* plan, const float* d_queries, const int64_t* d_offsets,
+                     const int32_t* d_lens, dtwax_result* d_out, void* stream);
+/* Enable HIP-event timing of the sDTW kernel launch inside dtwax_score_plan;
+ * dtwax_plan_kernel_ms then waits for and returns the last launch's time. */
+int dtwax_plan_set_timing(dtwax_plan* plan, int32_t enable);
+int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
+
+/* Mirror of FullDTW::distances (src/include/DTW.hpp:285-295) for the
+ * reference's fixed geometry (QUERY_LEN 2048, PREFIX_LEN 1024, SEGMENT 26,
+ * WARP 32, REF_LEN 48128, -DFP16 -DNO_REF_DEL): ref = REF_LEN half2 {fwd, rev}
+ * in the reference's tiled layout (src/main.cu:92-109); query = num_entries *
+ * 2048 half2 {q, q} plus 32 pad entries; dists = num_entries half2 {fwd, rev}.
+ * thresh is accepted and unused, as in the reference (DTW.cu:152-153);
+ * device_last_row may be NULL (the scratch is internal).  compat_quirks
+ * selects bit-exact reproduction of the reference kernel's seam deviations
+ * (1) or the clean recurrence (0).  All pointers are device pointers. */
+int dtwax_distances(dtwax_ctx* ctx, const void* ref, const void* query, void* dists,
+                    int32_t num_entries, uint32_t thresh, void* stream,
+                    void* device_last_row, int32_t compat_quirks);
+
+/* ---- self-test of the lane-movement primitives the kernels rely on ------
+ * Writes, for each of the 64 lanes, the value received by a DPP wave shift
+ * right by one (out[0..63]) and left by one (out[64..127]) of the lane
+ * index, with out-of-range lanes keeping -1.  Synchronous. */
+int dtwax_selftest_lanes(int32_t* out_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

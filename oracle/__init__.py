@@ -51,7 +51,7 @@ def lib():
                                    C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.orc_sdtw_f32.argtypes = [_f32p, C.c_int64, _f32p, C.c_int64, C.c_void_p,
                                    C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int64)]
-        for n in ("orc_sdtw_f16", "orc_sdtw_f16_fused"):
+        for n in ("orc_sdtw_f16", "orc_sdtw_f16_fused", "orc_sdtw_f16_scalar"):
             getattr(L, n).argtypes = [_u16p, C.c_int64, _u16p, C.c_int64, C.c_void_p,
                                       C.c_void_p, C.POINTER(C.c_uint16), C.POINTER(C.c_int64)]
         L.orc_sdtw_f64_trace.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _i64p, C.c_int, _f64p]
@@ -127,14 +127,15 @@ def sdtw_f32(q, s, init_row=None, want_row=False):
     return (sc.value, am.value, row) if want_row else (sc.value, am.value)
 
 
-def sdtw_f16(q_h, s_h, init_row=None, want_row=False, fused=False):
+def sdtw_f16(q_h, s_h, init_row=None, want_row=False, fused=False, scalar=False):
     """q_h, s_h: binary16 bit patterns (uint16).  Returns (score_bits, argmin[, row])."""
     q = np.ascontiguousarray(q_h, np.uint16)
     s = np.ascontiguousarray(s_h, np.uint16)
     init = None if init_row is None else np.ascontiguousarray(init_row, np.uint16)
     row = np.empty(s.size, np.uint16) if want_row else None
     sc, am = C.c_uint16(), C.c_int64()
-    fn = lib().orc_sdtw_f16_fused if fused else lib().orc_sdtw_f16
+    fn = (lib().orc_sdtw_f16_fused if fused else
+          lib().orc_sdtw_f16_scalar if scalar else lib().orc_sdtw_f16)
     fn(q, q.size, s, s.size, _ptr(init), _ptr(row), C.byref(sc), C.byref(am))
     return (sc.value, am.value, row) if want_row else (sc.value, am.value)
 

@@ -56,6 +56,11 @@ void orc_sdtw_f32(const float* q, int64_t Q, const float* s, int64_t N,
 void orc_sdtw_f16(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N,
                   const uint16_t* init_row, uint16_t* last_row,
                   uint16_t* score, int64_t* argmin);
+/* Same as orc_sdtw_f16, one cell at a time without SIMD conversions (used to
+ * cross-check the vectorised path). */
+void orc_sdtw_f16_scalar(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N,
+                         const uint16_t* init_row, uint16_t* last_row,
+                         uint16_t* score, int64_t* argmin);
 /* binary16 fused form: t = rn(s-q); D = rn(t*t + min) (one rounding). */
 void orc_sdtw_f16_fused(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N,
                         const uint16_t* init_row, uint16_t* last_row,

@@ -182,9 +182,66 @@ static void sdtw_f16_impl(const uint16_t* q, int64_t Q, const uint16_t* s,
   free(b);
 }
 
+#if defined(__F16C__) && defined(__AVX2__)
+#include <immintrin.h>
+/* Same arithmetic as sdtw_f16_impl(fused = 0), eight columns at a time with
+ * the x86 F16C conversions (IEEE round-to-nearest-even, subnormals kept).
+ * Cross-checked against the scalar path by tests/test_oracle_sdtw.py. */
+static void sdtw_f16_vec(const uint16_t* q, int64_t Q, const uint16_t* s,
+                         int64_t N, const uint16_t* init_row, uint16_t* last_row,
+                         uint16_t* score, int64_t* argmin) {
+  h2f_init();
+  const int64_t NP = (N + 7) / 8 * 8;
+  /* a[0] = column -1 (+inf); a[c+1] = column c; padded to NP+8 */
+  uint16_t* a = (uint16_t*)aligned_alloc(32, sizeof(uint16_t) * (size_t)(NP + 16));
+  uint16_t* b = (uint16_t*)aligned_alloc(32, sizeof(uint16_t) * (size_t)(NP + 16));
+  uint16_t* sp = (uint16_t*)aligned_alloc(32, sizeof(uint16_t) * (size_t)(NP + 16));
+  const uint16_t HINF = 0x7c00;
+  for (int64_t c = 0; c < NP + 16; c++) { a[c] = HINF; b[c] = HINF; sp[c] = 0; }
+  for (int64_t c = 0; c < N; c++) { a[c + 1] = init_row ? init_row[c] : 0; sp[c] = s[c]; }
+  const int R = _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC;
+  for (int64_t r = 0; r < Q; r++) {
+    const __m256 qv = _mm256_set1_ps(g_h2f[q[r]]);
+    b[0] = HINF;
+    for (int64_t c = 0; c < NP; c += 8) {
+      __m256 sv = _mm256_cvtph_ps(_mm_loadu_si128((const __m128i*)(sp + c)));
+      __m256 t = _mm256_cvtph_ps(_mm256_cvtps_ph(_mm256_sub_ps(sv, qv), R));
+      __m256 t2 = _mm256_cvtph_ps(_mm256_cvtps_ph(_mm256_mul_ps(t, t), R));
+      __m256 top = _mm256_cvtph_ps(_mm_loadu_si128((const __m128i*)(a + c + 1)));
+      __m256 dg = _mm256_cvtph_ps(_mm_loadu_si128((const __m128i*)(a + c)));
+      __m256 m = _mm256_min_ps(top, dg);
+      __m128i v = _mm256_cvtps_ph(_mm256_add_ps(t2, m), R);
+      _mm_storeu_si128((__m128i*)(b + c + 1), v);
+    }
+    uint16_t* tmp = a; a = b; b = tmp;
+  }
+  float best = INFINITY;
+  uint16_t bh = HINF;
+  int64_t bi = -1;
+  for (int64_t c = 0; c < N; c++)
+    if (g_h2f[a[c + 1]] < best) { best = g_h2f[a[c + 1]]; bh = a[c + 1]; bi = c; }
+  if (last_row) memcpy(last_row, a + 1, sizeof(uint16_t) * (size_t)N);
+  *score = bh;
+  *argmin = bi;
+  free(a);
+  free(b);
+  free(sp);
+}
+#endif
+
 void orc_sdtw_f16(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N,
                   const uint16_t* init_row, uint16_t* last_row, uint16_t* score,
                   int64_t* argmin) {
+#if defined(__F16C__) && defined(__AVX2__)
+  sdtw_f16_vec(q, Q, s, N, init_row, last_row, score, argmin);
+#else
+  sdtw_f16_impl(q, Q, s, N, init_row, last_row, score, argmin, 0);
+#endif
+}
+
+void orc_sdtw_f16_scalar(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N,
+                         const uint16_t* init_row, uint16_t* last_row,
+                         uint16_t* score, int64_t* argmin) {
   sdtw_f16_impl(q, Q, s, N, init_row, last_row, score, argmin, 0);
 }
 void orc_sdtw_f16_fused(const uint16_t* q, int64_t Q, const uint16_t* s,
@@ -237,8 +294,12 @@ void orc_batch_f16(const uint16_t* q, const int64_t* offsets,
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t i = 0; i < n_reads * n_strands; i++) {
     int64_t r = i / n_strands, k = i % n_strands;
-    sdtw_f16_impl(q + offsets[r], lens[r], s + k * N, N, NULL, NULL, &score[i],
-                  &argmin[i], fused);
+    if (fused)
+      sdtw_f16_impl(q + offsets[r], lens[r], s + k * N, N, NULL, NULL, &score[i],
+                    &argmin[i], 1);
+    else
+      orc_sdtw_f16(q + offsets[r], lens[r], s + k * N, N, NULL, NULL, &score[i],
+                   &argmin[i]);
   }
 }
 

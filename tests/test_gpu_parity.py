@@ -1,0 +1,182 @@
+"""GPU parity: the HIP kernels through the C ABI against the CPU oracle.
+
+Bar: bit-exact scores and argmins for every arithmetic form the oracle
+restates (binary16 per-op, binary16 fused, binary32), for every segment
+width and reference chunking; FP64-relative tolerance for the renormalised
+form (its offsets change the rounding by design).
+"""
+import numpy as np
+import pytest
+
+import dtwax_amd as D
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LENS = [1, 2, 63, 64, 65, 100, 511, 512, 513, 1000, 1599, 1600, 2047, 2048, 2049, 2500, 4097]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = D.Context(0)
+    yield c
+    c.close()
+
+
+def _half_vals(x):
+    return O.f16_to_f32(O.f32_to_f16(x)).reshape(np.shape(x))
+
+
+def test_lane_primitives():
+    shr, shl = D.selftest_lanes()
+    assert list(shr) == [-1] + list(range(63)), "DPP wave_shr:1 must move lane L-1 -> L"
+    assert list(shl) == list(range(1, 64)) + [-1], "DPP wave_shl:1 must move lane L+1 -> L"
+
+
+def _oracle_f16(reads, ref, fused=False):
+    qs = [O.f32_to_f16(r) for r in reads]
+    sh = np.stack([O.f32_to_f16(ref[k]) for k in range(ref.shape[0])])
+    sc, am = O.batch_f16(qs, sh, fused=fused)
+    return O.f16_to_f32(sc.ravel()).reshape(sc.shape), am
+
+
+@pytest.mark.parametrize("segment", [8, 16, 25, 32])
+def test_f16_bit_exact(ctx, segment):
+    rng = np.random.default_rng(segment)
+    ref = rng.standard_normal((2, 3001)).astype(np.float32)
+    ctx.upload_reference(ref)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in LENS]
+    sc, am = ctx.score(reads, precision="f16", segment=segment)
+    osc, oam = _oracle_f16(reads, ref)
+    assert np.array_equal(sc, osc)
+    assert np.array_equal(am, oam)
+
+
+def test_f16_fused_bit_exact(ctx):
+    rng = np.random.default_rng(41)
+    ref = rng.standard_normal((2, 900)).astype(np.float32)
+    ctx.upload_reference(ref)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in (1, 64, 130, 700)]
+    sc, am = ctx.score(reads, precision="f16_fused", segment=8)
+    osc, oam = _oracle_f16(reads, ref, fused=True)
+    assert np.array_equal(sc, osc) and np.array_equal(am, oam)
+
+
+@pytest.mark.parametrize("nstr", [1, 2])
+def test_f32_bit_exact(ctx, nstr):
+    rng = np.random.default_rng(50 + nstr)
+    ref = rng.standard_normal((nstr, 2500)).astype(np.float32)
+    ctx.upload_reference(ref)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in LENS[:-2]]
+    for seg in (8, 16):
+        sc, am = ctx.score(reads, precision="f32", segment=seg)
+        osc, oam = O.batch_f32(reads, ref)
+        assert np.array_equal(sc[:, :nstr], osc)
+        assert np.array_equal(am[:, :nstr], oam)
+        if nstr == 1:
+            assert np.all(am[:, 1] == -1)
+
+
+@pytest.mark.parametrize("prec", ["f16", "f32"])
+def test_reference_chunking_is_exact(ctx, prec):
+    """Chunks overlapping by (Q-1) columns give the unchunked answer exactly."""
+    rng = np.random.default_rng(61)
+    ref = rng.standard_normal((2, 20000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in (5, 300, 1100, 2100)]
+    full = ctx.score(reads, precision=prec, ref_chunk=0)
+    for chunk in (997, 4096, 7000):
+        part = ctx.score(reads, precision=prec, ref_chunk=chunk)
+        assert np.array_equal(full[0], part[0]) and np.array_equal(full[1], part[1])
+
+
+def test_exact_substring_scores_zero(ctx):
+    rng = np.random.default_rng(62)
+    ref = _half_vals(rng.standard_normal((2, 5000)).astype(np.float32)).reshape(2, 5000)
+    ctx.upload_reference(ref)
+    q = ref[1, 1234:1234 + 1777].copy()
+    sc, am = ctx.score([q], precision="f16")
+    assert sc[0, 1] == 0.0 and am[0, 1] == 1234 + 1776
+
+
+def test_renorm_tracks_fp64(ctx):
+    """The renormalised binary16 form stays within 2% of FP64 at Q = 8000 and
+    is not worse than plain binary16 on average."""
+    rng = np.random.default_rng(63)
+    ref = rng.standard_normal((2, 6000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    reads = [rng.standard_normal(8000).astype(np.float32) for _ in range(6)]
+    ex, _ = O.batch_f64([r.astype(np.float64) for r in reads], _half_vals(ref).reshape(2, -1).astype(np.float64))
+    f16, _ = ctx.score(reads, precision="f16")
+    ren, _ = ctx.score(reads, precision="f16_renorm")
+    e16 = np.abs(f16 - ex) / ex
+    ere = np.abs(ren - ex) / ex
+    assert np.all(ere < 0.02), ere
+    assert ere.mean() <= e16.mean() + 1e-3
+
+
+def test_normalize_medmad_exact(ctx):
+    rng = np.random.default_rng(64)
+    reads = [(rng.standard_normal(n) * 7 + 90).astype(np.float32) for n in (1, 2, 3, 1000, 2000, 4001)]
+    out, st = ctx.normalize(reads, mode=D.NORM_MEDMAD)
+    for r, y, (c, s) in zip(reads, out, st):
+        want, med, mad = O.norm_medmad(r)
+        assert c == med and np.isclose(s, 1.4826 * mad, rtol=0, atol=0)
+        assert np.array_equal(y, want)
+
+
+def test_normalize_meanstd(ctx):
+    rng = np.random.default_rng(65)
+    reads = [(rng.standard_normal(n) * 13 + 500).astype(np.float32) for n in (2, 1000, 2048)]
+    out, st = ctx.normalize(reads, mode=D.NORM_MEANSTD, eps=1e-4)
+    for r, y in zip(reads, out):
+        want, m, s = O.norm_meanstd(r, 1e-4)
+        # FP64 sums in a different order: at most one float ulp apart
+        assert np.all(np.abs(y - want) <= np.spacing(np.abs(want)))
+
+
+def test_distances_mirror_clean(ctx):
+    """dtwax_distances (FullDTW::distances mirror, reference layouts) with the
+    seam deviations off equals the fixed emulator, i.e. the clean binary16
+    recurrence over the 57 scored tiles."""
+    rng = np.random.default_rng(66)
+    REF_LEN, TILE, SEG, W = 48128, 832, 26, 32
+    flat = _half_vals(rng.standard_normal((2, REF_LEN)).astype(np.float32)).reshape(2, REF_LEN)
+    # tiled layout h[l*TILE + i*W + j] = flat[l*TILE + j*SEG + i] (main.cu:92-109)
+    tiled = np.zeros((REF_LEN, 2), np.float16)
+    for l in range(REF_LEN // TILE):
+        for i in range(SEG):
+            for k in range(2):
+                tiled[l * TILE + i * W:l * TILE + i * W + W, k] = flat[k, l * TILE + np.arange(W) * SEG + i]
+    n = 3
+    qs = rng.standard_normal((n, 2048)).astype(np.float32)
+    qh2 = np.zeros((n * 2048 + 32, 2), np.float16)
+    qh2[: n * 2048, 0] = qs.ravel()
+    qh2[: n * 2048, 1] = qs.ravel()
+    d_ref = D.DeviceArray.from_host(tiled.view(np.uint32).ravel())
+    d_q = D.DeviceArray.from_host(qh2.view(np.uint32).ravel())
+    d_out = D.DeviceArray((n,), np.uint32)
+    ctx.distances(d_ref, d_q, d_out, n, compat_quirks=0)
+    D.lib().dtwax_device_sync()
+    got = d_out.to_host().view(np.float16).reshape(n, 2).astype(np.float32)
+    want = O.emulate_reference(flat, qs, fix_q1=True, fix_q2=True)
+    assert np.array_equal(got, want)
+
+
+def test_c2_subset_decisions(ctx):
+    """64 reads of the 48.5 kb synthetic workload (3000 raw -> trim 1000 ->
+    2000, median/MAD on the GPU): binary16 scores and argmins bit-exact with
+    the oracle, hence identical decisions at any threshold."""
+    from dtwax_amd import synth
+    w = synth.config_workload(seed=2, ref_bp=48_500, n_reads=64, raw_len=3000, target_frac=0.5)
+    reads = synth.trim_reads(w.reads, 1000)
+    norm, _ = ctx.normalize(reads, mode=D.NORM_MEDMAD)
+    ctx.upload_reference(w.ref_levels)
+    sc, am = ctx.score(norm, precision="f16")
+    osc, oam = _oracle_f16(norm, w.ref_levels)
+    assert np.array_equal(sc, osc) and np.array_equal(am, oam)
+    best = sc.min(axis=1)
+    t = np.median(best)
+    assert np.array_equal(best < t, osc.min(axis=1) < t)
+    # the synthetic targets separate from the non-targets
+    assert best[w.is_target].max() < best[~w.is_target].min()

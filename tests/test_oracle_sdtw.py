@@ -86,6 +86,18 @@ def test_f32_and_f16_per_cell(seed):
     assert am == int(np.argmin(D16[-1]))
 
 
+def test_f16_vector_path_equals_scalar():
+    rng = np.random.default_rng(9)
+    for Q, N, scale in ((1, 7, 1.0), (37, 1001, 1.0), (300, 3000, 3.0), (200, 777, 1e-3)):
+        q = O.f32_to_f16(rng.standard_normal(Q) * scale)
+        s = O.f32_to_f16(rng.standard_normal(N) * scale)
+        init = O.f32_to_f16(np.abs(rng.standard_normal(N)) * 50)
+        for ir in (None, init):
+            a = O.sdtw_f16(q, s, init_row=ir, want_row=True)
+            b = O.sdtw_f16(q, s, init_row=ir, want_row=True, scalar=True)
+            assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2])
+
+
 def test_f16_fused_close_and_distinct():
     rng = np.random.default_rng(7)
     q = O.f32_to_f16(rng.standard_normal(400))
