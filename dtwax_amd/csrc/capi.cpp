@@ -289,7 +289,9 @@ int dtwax_plan_create(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   p->n_reads = n_reads;
   p->precision = o.precision;
   p->nstr = c->n_strands;
-  p->S = o.segment ? o.segment : auto_segment(lens);
+  // the renormalised form subtracts each band's exact row minimum, so it
+  // defaults to the shortest bands (64 x 8 rows)
+  p->S = o.segment ? o.segment : (o.precision == DTWAX_F16_RENORM ? 8 : auto_segment(lens));
   const float drift = o.renorm_drift > 0 ? o.renorm_drift : 0.1f;
   p->renorm_k = drift * (float)p->S;
   const int64_t N = c->n_cols;

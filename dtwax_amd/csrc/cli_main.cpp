@@ -1,0 +1,356 @@
+// cli_main.cpp -- drop-in command line driver.
+//
+//   dtwax_main <fast5_folder/> <kmer_model> <ref.fasta> [options]
+//
+// Same positional arguments, read selection and stdout log as the
+// reference's offline driver (src/main.cu:53-318; log format in its
+// TIMERSTOP / TIMERSTOP_CUDA macros, src/include/hpc_helpers.hpp:62-100):
+//   * reference: the last FASTA entry (earlier entries are printed as
+//     "name : content", load_reference.hpp:164-212), reverse complement,
+//     squiggle over the first REF_LEN k-mers per strand;
+//   * reads: every *.fast5 in readdir order, every read group in name order;
+//     reads shorter than ADAPTER_LEN + QUERY_LEN are skipped; samples
+//     [ADAPTER_LEN, ADAPTER_LEN + QUERY_LEN) are kept as raw ADC values;
+//   * mean/std normalisation (eps 1e-4), binary16 sDTW of both strands;
+//   * table: j, read group, QUERY_LEN, REF_LEN, fwd score, rev score.
+// Defaults reproduce the reference's compile-time geometry (QUERY_LEN 2048,
+// ADAPTER_LEN 1952, REF_LEN 48128 of which 57*832 = 47,424 columns are
+// scored).  Options after the positionals select the native mode (whole
+// reference, any query length, medmad, other precisions).
+// Exit status is non-zero on any error (the reference exits on the first
+// CUDA error and on a missing VBZ plugin).
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "dtwax.h"
+#include "fast5_reader.h"
+
+namespace {
+
+struct Options {
+  std::string dir, model, fasta;
+  int query_len = 2048;      // QUERY_LEN (common.hpp:65)
+  int adapter_len = 1952;    // ADAPTER_LEN (common.hpp:82)
+  int64_t ref_len = 48128;   // REF_LEN per strand (common.hpp:75, FP16 build)
+  int64_t scored_cols = 47424;  // (REF_LEN / (26*32)) * 26*32 (common.hpp:87-88)
+  bool native = false;       // all k-mers, all columns
+  int precision = DTWAX_F16;
+  int norm = DTWAX_NORM_MEANSTD;
+  double eps = 1e-4;
+  bool dump_reads = false;   // print loaded raw samples and stop (no GPU)
+  int device = 0;
+  bool fp32_build = false;   // the reference's FP32 build: concatenated fwd|rev, one score
+};
+
+void usage() {
+  std::cerr << "usage: dtwax_main <fast5_folder/> <kmer_model> <ref.fasta> [--native] [--query-len N]\n"
+               "       [--adapter-len N] [--precision f16|f16_fused|f16_renorm|f32] [--norm meanstd|medmad]\n"
+               "       [--fp32-build] [--device N] [--dump-reads]\n";
+}
+
+bool parse(int argc, char** argv, Options& o) {
+  if (argc < 4) return false;
+  o.dir = argv[1];
+  o.model = argv[2];
+  o.fasta = argv[3];
+  for (int i = 4; i < argc; i++) {
+    std::string a = argv[i];
+    auto next = [&](const char* what) -> const char* {
+      if (i + 1 >= argc) {
+        std::cerr << "missing value for " << what << "\n";
+        std::exit(2);
+      }
+      return argv[++i];
+    };
+    if (a == "--native") o.native = true;
+    else if (a == "--query-len") o.query_len = std::atoi(next("--query-len"));
+    else if (a == "--adapter-len") o.adapter_len = std::atoi(next("--adapter-len"));
+    else if (a == "--precision") {
+      std::string p = next("--precision");
+      if (p == "f16") o.precision = DTWAX_F16;
+      else if (p == "f16_fused") o.precision = DTWAX_F16_FUSED;
+      else if (p == "f16_renorm") o.precision = DTWAX_F16_RENORM;
+      else if (p == "f32") o.precision = DTWAX_F32;
+      else return false;
+    } else if (a == "--norm") {
+      std::string p = next("--norm");
+      if (p == "meanstd") o.norm = DTWAX_NORM_MEANSTD;
+      else if (p == "medmad") o.norm = DTWAX_NORM_MEDMAD;
+      else return false;
+    } else if (a == "--fp32-build") {
+      o.fp32_build = true;
+      o.precision = DTWAX_F32;
+    } else if (a == "--device") o.device = std::atoi(next("--device"));
+    else if (a == "--dump-reads") o.dump_reads = true;
+    else if (a == "--help" || a == "-h") return false;
+    else if (i == 4 && a == "pipe") continue;  // tolerated like the RU driver's argv[3]
+    else {
+      std::cerr << "unknown option " << a << "\n";
+      return false;
+    }
+  }
+  if (o.query_len < 1 || o.adapter_len < 0) return false;
+  return true;
+}
+
+using Clock = std::chrono::steady_clock;
+struct Timer {
+  Clock::time_point t0 = Clock::now();
+  double ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+};
+void timing(const char* label, double ms) { std::cout << "TIMING: " << ms << " ms (" << label << ")" << std::endl; }
+
+// load_reference::ref_loader (load_reference.hpp:164-212)
+bool load_fasta(const std::string& path, std::string& seq) {
+  std::ifstream in(path);
+  if (!in.good()) {
+    std::cerr << "Error opening '" << path << "'. Bailing out." << std::endl;
+    return false;
+  }
+  std::string line, name, content;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line.empty() || line[0] == '>') {
+      if (!name.empty()) {
+        std::cout << name << " : " << content << std::endl;
+        name.clear();
+      }
+      if (!line.empty()) name = line.substr(1);
+      content.clear();
+    } else if (!name.empty()) {
+      if (line.find(' ') != std::string::npos) {
+        name.clear();
+        content.clear();
+      } else {
+        content += line;
+      }
+    }
+  }
+  if (name.empty()) {
+    std::cerr << "Invalid FASTA entry in '" << path << "'. Bailing out." << std::endl;
+    return false;
+  }
+  seq = content;
+  return true;
+}
+
+// load_reference::read_kmer_model (load_reference.hpp:131-161): "KMER mean sd"
+bool load_model(const std::string& path, int& k, std::vector<float>& means) {
+  std::ifstream in(path);
+  if (!in.good()) {
+    std::cerr << "Error opening k-mer model '" << path << "'" << std::endl;
+    return false;
+  }
+  std::string line;
+  std::map<std::string, float> m;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    std::string kmer;
+    float mean = 0, sd = 0;
+    if (!(ss >> kmer >> mean >> sd)) continue;
+    m.insert({kmer, mean});  // first occurrence wins, as std::map::insert
+  }
+  if (m.empty()) {
+    std::cerr << "empty k-mer model '" << path << "'" << std::endl;
+    return false;
+  }
+  k = (int)m.begin()->first.size();
+  if (k < 1 || k > 12) return false;
+  means.assign((size_t)1 << (2 * k), 0.0f);
+  for (auto& kv : m) {
+    if ((int)kv.first.size() != k) continue;
+    int code = 0;
+    bool ok = true;
+    for (char ch : kv.first) {
+      int b = ch == 'A' ? 0 : ch == 'C' ? 1 : ch == 'G' ? 2 : ch == 'T' ? 3 : -1;
+      if (b < 0) { ok = false; break; }
+      code = code * 4 + b;
+    }
+    if (ok) means[(size_t)code] = kv.second;
+  }
+  return true;
+}
+
+#define CHECK(x)                                                                  \
+  do {                                                                            \
+    int rc_ = (x);                                                                \
+    if (rc_ != DTWAX_OK) {                                                        \
+      std::cerr << "dtwax error " << rc_ << " at " #x ": " << dtwax_last_error() << std::endl; \
+      std::exit(1);                                                               \
+    }                                                                             \
+  } while (0)
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Options o;
+  if (!parse(argc, argv, o)) {
+    usage();
+    return 2;
+  }
+  std::cout << std::flush;
+
+  // ---- reference (load_target) --------------------------------------------
+  Timer t_target;
+  std::string seq;
+  int k = 6;
+  std::vector<float> means;
+  if (!load_fasta(o.fasta, seq) || !load_model(o.model, k, means)) return 1;
+  const int64_t n_kmers = (int64_t)seq.size() - k + 1;
+  int64_t ref_len = o.native ? n_kmers : o.ref_len;
+  if (ref_len < 1) {
+    std::cerr << "reference shorter than one k-mer" << std::endl;
+    return 1;
+  }
+  std::vector<float> levels((size_t)(2 * ref_len));
+  int64_t skipped = 0;
+  CHECK(dtwax_squiggle_build(seq.data(), (int64_t)seq.size(), k, means.data(), ref_len, levels.data(), &skipped));
+  int64_t n_cols = o.native ? ref_len : std::min<int64_t>(o.scored_cols, ref_len);
+  dtwax_ctx* ctx = nullptr;
+  if (!o.dump_reads) {
+    CHECK(dtwax_create(o.device, &ctx));
+    if (o.fp32_build) {
+      // FP32 build: one sequence fwd[0:REF_LEN) | rev[0:REF_LEN), tiles of 832
+      // over the concatenation (load_reference.hpp:35-37, common.hpp:70-73)
+      const int64_t total = 2 * ref_len;
+      const int64_t scored = o.native ? total : (total / 832) * 832;
+      std::vector<float> cat(levels.begin(), levels.begin() + scored);
+      CHECK(dtwax_reference_upload(ctx, cat.data(), 1, scored));
+      n_cols = scored;
+    } else {
+      std::vector<float> two((size_t)(2 * n_cols));
+      std::copy(levels.begin(), levels.begin() + n_cols, two.begin());
+      std::copy(levels.begin() + ref_len, levels.begin() + ref_len + n_cols, two.begin() + n_cols);
+      CHECK(dtwax_reference_upload(ctx, two.data(), 2, n_cols));
+    }
+  }
+  timing("load_target", t_target.ms());
+
+  // ---- reads (load_data) ---------------------------------------------------
+  Timer t_data;
+  std::cout << "cuDTW:: Loading ONT fast5 reads from " << o.dir << "\n";
+  bool opened = false;
+  std::vector<std::string> files = dtwax_io::list_fast5(o.dir, &opened);
+  const int64_t need = (int64_t)o.query_len + o.adapter_len;
+  std::vector<std::string> ids;
+  std::vector<float> raw;
+  int64_t invalid = 0, valid = 0;
+  for (const auto& f : files) {
+    std::vector<dtwax_io::Fast5Read> reads;
+    std::string err;
+    bool ok = dtwax_io::read_fast5_file(f, reads, err);
+    if (!err.empty()) std::cerr << err << std::endl;
+    if (!ok) continue;
+    for (auto& r : reads) {
+      if ((int64_t)r.samples.size() < need) {
+        invalid++;
+        continue;
+      }
+      valid++;
+      ids.push_back(r.group);
+      raw.insert(raw.end(), r.samples.begin() + o.adapter_len, r.samples.begin() + need);
+    }
+  }
+  std::cout << "cuDTW:: " << files.size() << " ONT fast5 files read\n"
+            << "cuDTW:: Short/invalid reads exempted:: " << invalid << std::endl;
+  std::cout << "cuDTW:: Valid reads (NUM_READS) loaded :: " << valid << std::endl;
+  if (o.dump_reads) {
+    for (size_t j = 0; j < ids.size(); j++) {
+      std::cout << ids[j];
+      for (int i = 0; i < o.query_len; i++) std::cout << " " << raw[j * o.query_len + i];
+      std::cout << "\n";
+    }
+    return 0;
+  }
+  const int32_t n = (int32_t)valid;
+  std::vector<int64_t> offs((size_t)n);
+  std::vector<int32_t> lens((size_t)n, o.query_len);
+  for (int32_t i = 0; i < n; i++) offs[(size_t)i] = (int64_t)i * o.query_len;
+
+  Timer t_norm;
+  void *d_raw = nullptr, *d_norm = nullptr, *d_off = nullptr, *d_len = nullptr, *d_out = nullptr;
+  const size_t nbytes = raw.size() * sizeof(float);
+  CHECK(dtwax_malloc(&d_raw, nbytes));
+  CHECK(dtwax_malloc(&d_norm, nbytes));
+  CHECK(dtwax_malloc(&d_off, offs.size() * 8 + 8));
+  CHECK(dtwax_malloc(&d_len, lens.size() * 4 + 4));
+  CHECK(dtwax_malloc(&d_out, (size_t)std::max(1, n) * sizeof(dtwax_result)));
+  if (n) {
+    CHECK(dtwax_memcpy_h2d(d_raw, raw.data(), nbytes, nullptr));
+    CHECK(dtwax_memcpy_h2d(d_off, offs.data(), offs.size() * 8, nullptr));
+    CHECK(dtwax_memcpy_h2d(d_len, lens.data(), lens.size() * 4, nullptr));
+    CHECK(dtwax_normalize(ctx, (const float*)d_raw, (const int64_t*)d_off, (const int32_t*)d_len, n, o.norm,
+                          o.eps, (float*)d_norm, nullptr, nullptr));
+    CHECK(dtwax_device_sync());
+  }
+  timing("normalizer_kernel", t_norm.ms());
+  std::cout << "Normalized data:\n";
+  timing("load_data", t_data.ms());
+  Timer t_malloc;
+  timing("malloc", t_malloc.ms());
+
+  // ---- scoring (concurrent_DTW_kernel_launch) -------------------------------
+  // The reference issues batches of BLOCK_NUM = 10 reads (main.cu:211-271); the
+  // batch lines are kept for log compatibility, the work is one launch.
+  const int block_num = 10;
+  const int batch_count = n / block_num;
+  std::cout << "Batch count: " << batch_count << " num_reads:" << n << "\n";
+  for (int b = 0; b <= batch_count; b++) {
+    int in_batch = block_num;
+    if (b == batch_count) {
+      if (n % block_num == 0) {
+        if (batch_count != 0) break;
+        in_batch = n;
+      } else {
+        in_batch = n % block_num;
+      }
+    }
+    if (in_batch == 0) continue;
+    std::cout << "Processing batch_id: " << b << "\n";
+    std::cout << "Issuing " << in_batch << " reads (blocks) from base addr:" << (int64_t)b * block_num * o.query_len
+              << " to stream_id 0\n";
+  }
+  Timer t_dtw;
+  std::vector<dtwax_result> res((size_t)n);
+  if (n) {
+    dtwax_score_opts opt{};
+    opt.precision = o.precision;
+    CHECK(dtwax_score(ctx, (const float*)d_norm, (const int64_t*)d_off, (const int32_t*)d_len, n, &opt,
+                      (dtwax_result*)d_out, nullptr));
+    CHECK(dtwax_memcpy_d2h(res.data(), d_out, res.size() * sizeof(dtwax_result), nullptr));
+  }
+  const double dtw_ms = t_dtw.ms();
+  std::cout << "TIMING: " << dtw_ms << " ms " << ((o.query_len / (dtw_ms * 1e3)) * n / 10)
+            << " Mbps (concurrent_DTW_kernel_launch)" << std::endl;
+
+  // ---- table (main.cu:279-297) ------------------------------------------------
+  const int64_t ref_len_printed = o.fp32_build ? 2 * ref_len : ref_len;
+  if (o.fp32_build) {
+    std::cout << "Read_ID\t" << "QUERY_LEN\t" << "REF_LEN\t" << "sDTW-score\n";
+    for (int32_t j = 0; j < n; j++)
+      std::cout << j << "\t" << ids[(size_t)j] << "\t" << o.query_len << "\t" << ref_len_printed << "\t"
+                << res[(size_t)j].score[0] << "\n";
+  } else {
+    std::cout << "Read_ID\t" << "QUERY_LEN\t" << "REF_LEN\t" << "sDTW score: fwd-strand\tsDTW score: rev-strand\n";
+    for (int32_t j = 0; j < n; j++)
+      std::cout << j << "\t" << ids[(size_t)j] << "\t" << o.query_len << "\t" << ref_len_printed << "\t"
+                << res[(size_t)j].score[0] << "\t" << res[(size_t)j].score[1] << "\n";
+  }
+  Timer t_free;
+  for (void* p : {d_raw, d_norm, d_off, d_len, d_out}) dtwax_free(p);
+  dtwax_destroy(ctx);
+  timing("free", t_free.ms());
+  return 0;
+}

@@ -80,6 +80,20 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// Work fetch and result publication without lane-divergent branches: every
+// lane issues the atomic, lane 0 with the real operand and the others with
+// the identity, and the wave takes lane 0's result.  (A lane-0-only branch
+// around the fetch inside the persistent loop lets the control-flow
+// structurizer split the loop so that lanes 1..63 re-read the broadcast
+// while lane 0 is masked off.)
+__device__ __forceinline__ int fetch_item(int32_t* counter, int lane) {
+  const int v = atomicAdd(counter, lane == 0 ? 1 : 0);
+  return __builtin_amdgcn_readlane(v, 0);
+}
+__device__ __forceinline__ void publish_min(unsigned long long* addr, uint64_t key, int lane) {
+  atomicMin(addr, lane == 0 ? (unsigned long long)key : ~0ull);
+}
+
 constexpr int RING = 256;        // ring slots
 constexpr int RING_ALLOC = 320;  // + mirror of slots [0, 64)
 
@@ -242,6 +256,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
   // minimum of the row the previous band left, per strand.
   float off_f = 0.0f, off_r = 0.0f;
   h2 prev_min = hsplat((half_t)0.0f);
+  h2 Kb = K;  // per-lane drift of the current band (adapted band to band)
 
   for (int32_t b = 0; b < nb; b++) {
     F16Band<S, MODE> B;
@@ -270,15 +285,15 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     io.c1 = it.c1;
     io.inj_lane = (b == 0) ? (P / S) : 0;
     h2 rowmin = hsplat((half_t)__builtin_inff());
-    h2 Kl = K;
-    if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = K + prev_min;
+    h2 Kl = Kb;
+    if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = Kb + prev_min;
     f16_sweep<S, MODE, INJ>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
     if (MODE == DTWAX_F16_RENORM) {
       // lanes whose subtraction lies on the path to the last row
       const int32_t first = INJ ? io.inj_lane : 0;
       const h2 k0 = __builtin_bit_cast(h2, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Kl), first));
-      off_f += (float)(63 - first) * (float)K.x + (float)k0.x;
-      off_r += (float)(63 - first) * (float)K.y + (float)k0.y;
+      off_f += (float)(63 - first) * (float)Kb.x + (float)k0.x;
+      off_r += (float)(63 - first) * (float)Kb.y + (float)k0.y;
       if (!io.final_band) {
         h2 m = rowmin;
 #pragma unroll
@@ -287,6 +302,9 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
           m = hmin2(m, o);
         }
         prev_min = m;
+        // The row minimum still rose by m over this band's 64 lane hops:
+        // refine the per-lane drift for the next band (per strand).
+        Kb = Kb + m * hsplat((half_t)(1.0f / 64.0f));
       }
     }
     __builtin_amdgcn_s_waitcnt(0);
@@ -296,12 +314,8 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
   // Combine: per strand min value (as float + offset), lowest column on ties.
   uint64_t kf = (posf >= 0) ? make_key((float)best.x + off_f, posf) : ~0ull;
   uint64_t kr = (posr >= 0) ? make_key((float)best.y + off_r, posr) : ~0ull;
-  kf = wave_min_u64(kf);
-  kr = wave_min_u64(kr);
-  if (lane == 0) {
-    if (kf != ~0ull) atomicMin((unsigned long long*)&p.keys[2 * (int64_t)read], (unsigned long long)kf);
-    if (kr != ~0ull) atomicMin((unsigned long long*)&p.keys[2 * (int64_t)read + 1], (unsigned long long)kr);
-  }
+  publish_min(&p.keys[2 * (int64_t)read], wave_min_u64(kf), lane);
+  publish_min(&p.keys[2 * (int64_t)read + 1], wave_min_u64(kr), lane);
 }
 
 template <int S, int MODE, bool INJ>
@@ -311,9 +325,7 @@ __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
   const int lane = threadIdx.x;
   h2* scratch = (h2*)p.scratch + (int64_t)blockIdx.x * p.scratch_stride + SCRATCH_PADL;
   for (;;) {
-    int idx = 0;
-    if (lane == 0) idx = atomicAdd(p.counter, 1);
-    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0, 64));
+    const int idx = fetch_item(p.counter, lane);
     if (idx >= p.n_items) break;
     WorkItem it = p.items[idx];
     f16_item<S, MODE, INJ>(p, it, lane, ring_s, ring_t, scratch);
@@ -474,9 +486,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
 #pragma unroll
   for (int st = 0; st < NSTR; st++) {
     uint64_t k = (pos[st] >= 0) ? make_key(best[st], pos[st]) : ~0ull;
-    k = wave_min_u64(k);
-    if (lane == 0 && k != ~0ull)
-      atomicMin((unsigned long long*)&p.keys[2 * (int64_t)read + st], (unsigned long long)k);
+    publish_min(&p.keys[2 * (int64_t)read + st], wave_min_u64(k), lane);
   }
 }
 
@@ -487,9 +497,7 @@ __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
   const int lane = threadIdx.x;
   float* scratch = (float*)p.scratch + ((int64_t)blockIdx.x * p.scratch_stride + SCRATCH_PADL) * NSTR;
   for (;;) {
-    int idx = 0;
-    if (lane == 0) idx = atomicAdd(p.counter, 1);
-    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0, 64));
+    const int idx = fetch_item(p.counter, lane);
     if (idx >= p.n_items) break;
     WorkItem it = p.items[idx];
     f32_item<S, NSTR, INJ>(p, it, lane, ring_s, ring_t, scratch);

@@ -77,11 +77,27 @@ class Workload:
     origin: np.ndarray = None                     # (strand, kmer start) per target read, -1 else
 
 
-def build_reference(rng, ref_bp, model_means, k=6):
+def medmad_z(x):
+    """(x - median) / (1.4826 * MAD) in float64, returned as float32."""
+    x = np.asarray(x, np.float64)
+    med = np.median(x)
+    mad = np.median(np.abs(x - med))
+    return ((x - med) / (1.4826 * mad)).astype(np.float32)
+
+
+def build_reference(rng, ref_bp, model_means, k=6, ref_norm="medmad"):
+    """ref_norm: "meanstd" -- the reference squiggle arithmetic of
+    load_reference.hpp:33-108 (dtwax_squiggle_build); "medmad" -- the same
+    robust estimator the reads get, per strand.  With the uniform level model
+    above, 1.4826*MAD is 1.28x the standard deviation, so mixing estimators
+    would put targets 28% out of scale."""
     from . import squiggle_build
     seq = random_seq(rng, ref_bp)
-    levels, skipped = squiggle_build(seq, model_means, None, k)
-    assert skipped == 0
+    if ref_norm == "meanstd":
+        levels, skipped = squiggle_build(seq, model_means, None, k)
+        assert skipped == 0
+    else:
+        levels = np.stack([medmad_z(model_means[kmer_codes(s, k)]) for s in (seq, revcomp(seq))])
     return seq, levels
 
 
@@ -110,11 +126,12 @@ def make_reads(rng, seq, model_means, lengths, target_frac, k=6):
     return reads, is_t, origin
 
 
-def config_workload(seed, ref_bp, n_reads, raw_len, target_frac=0.5, len_range=None):
+def config_workload(seed, ref_bp, n_reads, raw_len, target_frac=0.5, len_range=None,
+                    ref_norm="medmad"):
     """len_range=(lo, hi): uniform integer raw lengths; else all raw_len."""
     rng = np.random.default_rng(seed)
     means, _ = make_model(rng)
-    seq, levels = build_reference(rng, ref_bp, means)
+    seq, levels = build_reference(rng, ref_bp, means, ref_norm=ref_norm)
     if len_range is not None:
         lengths = rng.integers(len_range[0], len_range[1] + 1, n_reads)
     else:

@@ -1,0 +1,79 @@
+"""The drop-in command line driver: same arguments, read selection and log
+format as the reference's offline main (src/main.cu:53-318)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import GOLDEN, fasta_python, load_model_means
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "dtwax_amd", "bin", "dtwax_main")
+ARGS = [GOLDEN + "/", os.path.join(GOLDEN, "dna_kmer_model.txt"), os.path.join(GOLDEN, "reference.fasta")]
+
+
+def run(*extra, check=True):
+    p = subprocess.run([CLI, *ARGS, *extra], capture_output=True, text=True, timeout=300)
+    if check:
+        assert p.returncode == 0, p.stderr
+    return p
+
+
+def test_dump_reads_selection_and_trim():
+    """Valid reads keep samples [1952, 4000) as raw ADC values."""
+    p = run("--dump-reads")
+    lines = p.stdout.splitlines()
+    assert "cuDTW:: Loading ONT fast5 reads from " + GOLDEN + "/" in lines
+    assert "cuDTW:: 1 ONT fast5 files read" in lines
+    assert "cuDTW:: Short/invalid reads exempted:: 0" in lines
+    assert "cuDTW:: Valid reads (NUM_READS) loaded :: 1" in lines
+    row = [l for l in lines if l.startswith("read_")][0].split()
+    assert row[0] == "read_0083a4a2-b062-44ed-988e-e7f5a301121f"
+    sig = np.load(os.path.join(GOLDEN, "batch0_signal.npy"))
+    assert np.array_equal(np.array(row[1:], np.float64), sig[1952:4000].astype(np.float64))
+
+
+def test_short_reads_are_exempted():
+    p = run("--dump-reads", "--query-len", "4000")
+    assert "cuDTW:: Short/invalid reads exempted:: 1" in p.stdout.splitlines()
+    assert "cuDTW:: Valid reads (NUM_READS) loaded :: 0" in p.stdout.splitlines()
+
+
+def test_usage_and_errors():
+    p = subprocess.run([CLI], capture_output=True, text=True)
+    assert p.returncode == 2 and "usage" in p.stderr
+    p = subprocess.run([CLI, GOLDEN + "/", ARGS[1], "/nonexistent.fasta", "--dump-reads"],
+                       capture_output=True, text=True)
+    assert p.returncode == 1 and "Error opening" in p.stderr
+
+
+@pytest.mark.gpu
+def test_table_matches_oracle():
+    """Full run on the test read: the table rows carry the binary16 scores of
+    the clean recurrence over the 47,424 scored columns of the 48,128-k-mer
+    squiggle (mean/std normalised query, eps 1e-4)."""
+    p = run()
+    lines = p.stdout.splitlines()
+    hdr = "Read_ID\tQUERY_LEN\tREF_LEN\tsDTW score: fwd-strand\tsDTW score: rev-strand"
+    assert hdr in lines
+    assert any(re.match(r"TIMING: \S+ ms \S+ Mbps \(concurrent_DTW_kernel_launch\)", l) for l in lines)
+    rows = [l.split("\t") for l in lines if re.match(r"^\d+\tread_", l)]
+    assert len(rows) == 1 and rows[0][1] == "read_0083a4a2-b062-44ed-988e-e7f5a301121f"
+    assert rows[0][2] == "2048" and rows[0][3] == "48128"
+    fwd, rev = float(rows[0][4]), float(rows[0][5])
+    # oracle
+    _, means = load_model_means()
+    fa = fasta_python()
+    rc = fa[::-1].translate(str.maketrans("ACGT", "TGCA"))
+    lv, _ = O.squiggle_reference(fa, rc, means, 48128)
+    sig = np.load(os.path.join(GOLDEN, "batch0_signal.npy")).astype(np.float32)[1952:4000]
+    q, _, _ = O.norm_meanstd(sig, 1e-4)
+    qh = O.f32_to_f16(q)
+    want = []
+    for k in range(2):
+        sc, _ = O.sdtw_f16(qh, O.f32_to_f16(lv[k, :47424]))
+        want.append(float(O.f16_to_f32(np.array([sc], np.uint16))[0]))
+    assert float(f"{want[0]:g}") == fwd and float(f"{want[1]:g}") == rev, (fwd, rev, want)
