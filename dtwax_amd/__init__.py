@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdtwax.so")
+# DTWAX_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("DTWAX_LIB", os.path.join(_HERE, "libdtwax.so"))
 
 F16, F16_FUSED, F16_RENORM, F32 = 0, 1, 2, 3
 PRECISIONS = {"f16": F16, "f16_fused": F16_FUSED, "f16_renorm": F16_RENORM, "f32": F32}

@@ -107,24 +107,38 @@ struct F16Band {
 };
 
 // One step: Do = column c-1 values, Dn = column c values.
+//
+// gfx950 needs a wait state between v_pk_fma_f16 and a dependent VALU op.
+// The row costs t_i = fma(s, a_i, -q_i) are therefore issued one group of G
+// rows ahead of the dependent chain (t*t, min, add), with scheduling barriers
+// so the scheduler cannot pull a consumer back next to its fma.
 template <int S, int MODE>
 __device__ __forceinline__ void f16_cells(const F16Band<S, MODE>& B, h2 s, h2 tin, h2 dg,
                                           const h2 (&Do)[S], h2 (&Dn)[S]) {
+  constexpr int G = 4;
   h2 tp = tin, dd = dg;
+  h2 t[S];
 #pragma unroll
-  for (int i = 0; i < S; i++) {
-    h2 t = hfma2(s, B.A[i], B.Qn[i]);  // == rn(s - q) for a real row (a = 1)
-    h2 m = hmin2(tp, dd);
-    h2 v;
-    if constexpr (MODE == DTWAX_F16_FUSED) {
-      v = hfma2(t, t, m);
-    } else {
-      h2 c = t * t;
-      v = c + m;
+  for (int i = 0; i < G && i < S; i++) t[i] = hfma2(s, B.A[i], B.Qn[i]);
+#pragma unroll
+  for (int g0 = 0; g0 < S; g0 += G) {
+#pragma unroll
+    for (int i = g0 + G; i < g0 + 2 * G && i < S; i++) t[i] = hfma2(s, B.A[i], B.Qn[i]);
+#pragma unroll
+    for (int i = g0; i < g0 + G && i < S; i++) {
+      h2 m = hmin2(tp, dd);
+      h2 v;
+      if constexpr (MODE == DTWAX_F16_FUSED) {
+        v = hfma2(t[i], t[i], m);
+      } else {
+        h2 c = t[i] * t[i];
+        v = c + m;
+      }
+      Dn[i] = v;
+      dd = Do[i];
+      tp = v;
     }
-    Dn[i] = v;
-    dd = Do[i];
-    tp = v;
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
