@@ -41,7 +41,23 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the bounded cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="also run one plain device copy of 256 MiB (normaliser copy mode) "
+                         "before timing, as the byte-count calibration for HBM counters")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sdtw_f16_kernel_48kb_4096x2000_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per sDTW launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -82,6 +98,16 @@ def main():
         D._check(L.dtwax_normalize(ctx.handle, d_raw.ptr, d_off.ptr, d_len.ptr, len(raw),
                                    D.NORM_MEDMAD, 0.0, d_norm.ptr, None, stream))
         plan.run(d_norm, d_off, d_len, d_out, stream)
+
+    if a.traffic_probe:
+        n_copy = 64 << 20  # floats: 256 MiB read + 256 MiB written
+        d_a = D.DeviceArray((n_copy,), np.float32)
+        d_b = D.DeviceArray((n_copy,), np.float32)
+        d_o = D.DeviceArray.from_host(np.zeros(1, np.int64))
+        d_l = D.DeviceArray.from_host(np.array([n_copy], np.int32))
+        D._check(L.dtwax_normalize(ctx.handle, d_a.ptr, d_o.ptr, d_l.ptr, 1, D.NORM_NONE, 0.0,
+                                   d_b.ptr, None, stream))
+        D._check(L.dtwax_stream_sync(stream))
 
     for _ in range(a.warmup):
         step()
@@ -139,7 +165,9 @@ def main():
         "kernel_gcups": round(cell_rate / 1e9, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved_ops / 1e12, 2),
                      "peak": round(peak_ops / 1e12, 2), "unit": "TOP/s",
-                     "frac": round(achieved_ops / peak_ops, 4), "traffic": None,
+                     "frac": round(achieved_ops / peak_ops, 4),
+                     "traffic": (measured_traffic() or {}).get("hbm_bytes_per_launch"),
+                     "algorithmic_bytes": int(lens.sum()) * 4 + 2 * n_cols * 2 + len(raw) * 48,
                      "ops_per_cell": 3, "device": info},
         "checks": {"targets_separate": sep, "median_threshold": round(thr, 3)},
         "timing": {"wall_s": round(wall, 3), "gen_s": round(gen_s, 2)},
