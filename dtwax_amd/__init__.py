@@ -75,6 +75,18 @@ def lib():
             "dtwax_plan_kernel_ms": ([vp, C.POINTER(f32)], C.c_int),
             "dtwax_distances": ([vp, vp, vp, vp, i32, C.c_uint32, vp, vp, i32], C.c_int),
             "dtwax_selftest_lanes": ([vp], C.c_int),
+            "dtwax_ru_create": ([vp, vp, C.POINTER(vp)], C.c_int),
+            "dtwax_ru_destroy": ([vp], C.c_int),
+            "dtwax_ru_info": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)],
+                              C.c_int),
+            "dtwax_ru_stats": ([vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64),
+                                C.POINTER(C.c_double)], C.c_int),
+            "dtwax_ru_process": ([vp, vp, i32, vp, vp], C.c_int),
+            "dtwax_ru_release": ([vp, i32], C.c_int),
+            "dtwax_ru_start": ([vp], C.c_int),
+            "dtwax_ru_submit": ([vp, i32, i32, vp], C.c_int),
+            "dtwax_ru_poll": ([vp, vp, i32, C.POINTER(i32), i32], C.c_int),
+            "dtwax_ru_stop": ([vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -94,6 +106,8 @@ def exported_symbols():
         "dtwax_reference_upload", "dtwax_squiggle_build", "dtwax_normalize", "dtwax_score",
         "dtwax_plan_create", "dtwax_plan_destroy", "dtwax_plan_info", "dtwax_score_plan",
         "dtwax_plan_set_timing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
+        "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
+        "dtwax_ru_release", "dtwax_ru_start", "dtwax_ru_submit", "dtwax_ru_poll", "dtwax_ru_stop",
     ]
 
 
@@ -279,3 +293,97 @@ def selftest_lanes():
     out = np.empty(128, np.int32)
     _check(lib().dtwax_selftest_lanes(out.ctypes.data))
     return out[:64], out[64:]
+
+
+# ---- Read Until serving ----------------------------------------------------
+RU_CONTINUE, RU_NONTARGET, RU_TARGET, RU_NO_SLOT, RU_STALE = 0, 1, 2, 3, 4
+RU_ACTIONS = {RU_CONTINUE: "continue", RU_NONTARGET: "nontarget", RU_TARGET: "target",
+              RU_NO_SLOT: "no_slot", RU_STALE: "stale"}
+
+
+class RuOpts(C.Structure):
+    _fields_ = [("max_reads", C.c_int32), ("chunk_len", C.c_int32), ("max_chunks", C.c_int32),
+                ("precision", C.c_int32), ("segment", C.c_int32), ("norm", C.c_int32),
+                ("eps", C.c_double), ("threshold", C.c_float), ("min_ref_chunk", C.c_int32),
+                ("hbm_budget", C.c_int64), ("reserved", C.c_int32 * 4)]
+
+
+RU_CHUNK_DTYPE = np.dtype([("channel", np.int32), ("number", np.int32), ("offset", np.int64)])
+RU_DECISION_DTYPE = np.dtype([("channel", np.int32), ("number", np.int32), ("action", np.int32),
+                              ("chunks", np.int32), ("score", np.float32, 2),
+                              ("argmin", np.int32, 2), ("latency_us", np.float64)])
+
+
+class ReadUntil:
+    """Stateful chunked classifier (dtwax_ru_* in include/dtwax.h) on a
+    context's current reference."""
+
+    def __init__(self, ctx, max_reads=512, chunk_len=1600, max_chunks=4, precision=F16,
+                 segment=0, norm=NORM_MEDMAD, eps=0.0, threshold=0.0, min_ref_chunk=0,
+                 hbm_budget=0):
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        o = RuOpts(max_reads=max_reads, chunk_len=chunk_len, max_chunks=max_chunks,
+                   precision=precision, segment=segment, norm=norm, eps=eps,
+                   threshold=threshold, min_ref_chunk=min_ref_chunk, hbm_budget=hbm_budget)
+        p = C.c_void_p()
+        _check(lib().dtwax_ru_create(ctx.handle, C.byref(o), C.byref(p)))
+        self.handle = p.value
+        self.ctx = ctx  # the engine reads the context's reference
+        self.chunk_len = chunk_len
+
+    def info(self):
+        sl, act, seg, nb = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
+        _check(lib().dtwax_ru_info(self.handle, C.byref(sl), C.byref(act), C.byref(seg), C.byref(nb)))
+        return dict(slots=sl.value, active=act.value, segment=seg.value, state_bytes=nb.value)
+
+    def stats(self):
+        b, ch, ce, ms = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib().dtwax_ru_stats(self.handle, C.byref(b), C.byref(ch), C.byref(ce), C.byref(ms)))
+        return dict(batches=b.value, chunks=ch.value, cells=ce.value, gpu_ms=ms.value)
+
+    def process(self, channels, numbers, samples):
+        """One batch: samples (n, chunk_len) float32, one row per channel.
+        Returns a RU_DECISION_DTYPE array."""
+        samples = np.ascontiguousarray(samples, np.float32).reshape(-1, self.chunk_len)
+        n = samples.shape[0]
+        ch = np.zeros(n, RU_CHUNK_DTYPE)
+        ch["channel"] = channels
+        ch["number"] = numbers
+        ch["offset"] = np.arange(n, dtype=np.int64) * self.chunk_len
+        out = np.zeros(n, RU_DECISION_DTYPE)
+        _check(lib().dtwax_ru_process(self.handle, ch.ctypes.data, n, samples.ctypes.data,
+                                      out.ctypes.data))
+        return out
+
+    def release(self, channel):
+        _check(lib().dtwax_ru_release(self.handle, channel))
+
+    def start(self):
+        _check(lib().dtwax_ru_start(self.handle))
+
+    def submit(self, channel, number, samples):
+        samples = np.ascontiguousarray(samples, np.float32)
+        if samples.size != self.chunk_len:
+            raise ValueError(f"a chunk holds {self.chunk_len} samples")
+        _check(lib().dtwax_ru_submit(self.handle, channel, number, samples.ctypes.data))
+
+    def poll(self, cap=1024, timeout_ms=0):
+        out = np.zeros(cap, RU_DECISION_DTYPE)
+        n = C.c_int32()
+        _check(lib().dtwax_ru_poll(self.handle, out.ctypes.data, cap, C.byref(n), timeout_ms))
+        return out[: n.value]
+
+    def stop(self):
+        _check(lib().dtwax_ru_stop(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().dtwax_ru_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -16,52 +16,28 @@
 #include <vector>
 
 #include "dtwax.h"
+#include "internal.h"
 #include "kernels.h"
 
 using namespace dtwax;
 
 namespace {
-
 thread_local std::string g_err;
+inline hipStream_t S_(void* s) { return reinterpret_cast<hipStream_t>(s); }
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+namespace dtwax {
+int set_error(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+}  // namespace dtwax
 
-#define HIP_TRY(expr)                                                                     \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(DTWAX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
-  } while (0)
-
-inline hipStream_t S_(void* s) { return reinterpret_cast<hipStream_t>(s); }
-
+namespace {
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
 }  // namespace
 
-struct dtwax_ctx {
-  int device = 0;
-  int n_cu = 0;
-  int clock_khz = 0;
-  std::string name;
-  // reference
-  int32_t n_strands = 0;
-  int64_t n_cols = 0;
-  void* ref16_alloc = nullptr;  // h2 per column, COL_PADL + n + COL_PADR
-  void* ref32_alloc = nullptr;  // n_strands floats per column, same padding
-  // compat entry point buffers
-  void* compat_ref = nullptr;
-  float* compat_q = nullptr;
-  int64_t compat_q_cap = 0;
-  int64_t* compat_off = nullptr;
-  int32_t* compat_len = nullptr;
-  DeviceResult* compat_res = nullptr;
-  int32_t compat_cap = 0;
-  dtwax_plan* compat_plan = nullptr;
-  int32_t compat_plan_n = -1;
-  int32_t compat_plan_quirks = -1;
-};
+#define HIP_TRY(expr) DTWAX_HIP_TRY(expr)
 
 struct dtwax_plan {
   dtwax_ctx* ctx = nullptr;
@@ -244,9 +220,13 @@ int dtwax_normalize(dtwax_ctx* c, const float* d_raw, const int64_t* d_off, cons
 }
 
 // ---- planning ------------------------------------------------------------
-static int occupancy(int precision, int S, int nstr, bool inj) {
+}  // extern "C"
+
+int dtwax::occupancy(int precision, int S, int nstr, bool inj) {
   return precision == DTWAX_F32 ? occupancy_f32(S, nstr, inj) : occupancy_f16(precision, S, inj);
 }
+
+extern "C" {
 
 static bool valid_segment(int S) { return S == 8 || S == 16 || S == 25 || S == 32; }
 
@@ -299,7 +279,7 @@ int dtwax_plan_create(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   bool multiband = false;
   for (int32_t q : lens) multiband |= q > BR;
 
-  const int occ = std::max(1, occupancy(p->precision, p->S, p->nstr, false));
+  const int occ = std::max(1, dtwax::occupancy(p->precision, p->S, p->nstr, false));
   const int64_t slots = (int64_t)occ * c->n_cu;
 
   // chunking: explicit, or whole strands unless the GPU would be underfilled
@@ -417,49 +397,73 @@ int dtwax_plan_kernel_ms(dtwax_plan* p, float* ms) {
   return DTWAX_OK;
 }
 
-static int run_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const int32_t* d_len,
-                    dtwax_result* d_out, hipStream_t st, const void* row_in, void* row_out,
-                    int64_t row_stride, const WorkItem* items_override, int32_t n_items_override) {
-  dtwax_ctx* c = p->ctx;
+}  // extern "C"
+
+namespace dtwax {
+int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
   HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(launch_reset(p->d_counter, p->d_keys, (int64_t)p->n_reads * 2, st));
+  HIP_TRY(launch_reset(L.counter, L.keys, (int64_t)L.n_reads * 2, st));
   KParams k{};
-  k.ref = (p->precision == DTWAX_F32)
+  k.ref = (L.precision == DTWAX_F32)
               ? (const void*)((const float*)c->ref32_alloc + (int64_t)COL_PADL * c->n_strands)
               : (const void*)((const uint32_t*)c->ref16_alloc + COL_PADL);
   k.n_cols = (int32_t)c->n_cols;
-  k.n_items = items_override ? n_items_override : p->n_items;
-  k.queries = d_q;
-  k.q_off = d_off;
-  k.q_len = d_len;
-  k.items = items_override ? items_override : p->d_items;
-  k.counter = p->d_counter;
-  k.keys = p->d_keys;
-  k.scratch = p->d_scratch;
-  k.scratch_stride = p->scratch_stride;
-  k.row_in = row_in;
-  k.row_out = row_out;
-  k.row_stride = row_stride;
-  k.renorm_k = p->renorm_k;
-  const int grid = items_override ? std::max(1, std::min(p->grid, n_items_override)) : p->grid;
+  k.n_items = L.n_items;
+  k.queries = L.queries;
+  k.q_off = L.q_off;
+  k.q_len = L.q_len;
+  k.items = L.items;
+  k.counter = L.counter;
+  k.keys = L.keys;
+  k.scratch = L.scratch;
+  k.scratch_stride = L.scratch_stride;
+  k.state = L.state;
+  k.row_stride = L.row_stride;
+  k.renorm_k = L.renorm_k;
   if (k.n_items > 0) {
-    if (p->timing) HIP_TRY(hipEventRecord(p->ev0, st));
-    HIP_TRY(launch_sdtw(k, p->precision, p->S, p->nstr, grid, p->inj, st));
-    if (p->timing) {
-      HIP_TRY(hipEventRecord(p->ev1, st));
-      p->timed_once = true;
-    }
+    if (L.ev0) HIP_TRY(hipEventRecord(L.ev0, st));
+    HIP_TRY(launch_sdtw(k, L.precision, L.S, L.nstr, std::max(1, std::min(L.grid, L.n_items)), false, st));
+    if (L.ev1) HIP_TRY(hipEventRecord(L.ev1, st));
   }
-  HIP_TRY(launch_finalize(p->d_keys, p->n_reads, p->precision == DTWAX_F32 ? p->nstr : 2,
-                          reinterpret_cast<DeviceResult*>(d_out), st));
+  HIP_TRY(launch_finalize(L.keys, L.n_reads, L.precision == DTWAX_F32 ? L.nstr : 2, L.out, st));
   return DTWAX_OK;
+}
+}  // namespace dtwax
+
+extern "C" {
+
+static int run_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const int32_t* d_len,
+                    dtwax_result* d_out, hipStream_t st) {
+  ScoreLaunch L;
+  L.precision = p->precision;
+  L.S = p->S;
+  L.nstr = p->nstr;
+  L.grid = p->grid;
+  L.queries = d_q;
+  L.q_off = d_off;
+  L.q_len = d_len;
+  L.items = p->d_items;
+  L.n_items = p->n_items;
+  L.n_reads = p->n_reads;
+  L.counter = p->d_counter;
+  L.keys = p->d_keys;
+  L.scratch = p->d_scratch;
+  L.scratch_stride = p->scratch_stride;
+  L.renorm_k = p->renorm_k;
+  L.out = reinterpret_cast<DeviceResult*>(d_out);
+  if (p->timing) {
+    L.ev0 = p->ev0;
+    L.ev1 = p->ev1;
+    p->timed_once = p->n_items > 0;
+  }
+  return run_score_launch(p->ctx, L, st);
 }
 
 int dtwax_score_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const int32_t* d_len,
                      dtwax_result* d_out, void* st) {
   if (!p || (p->n_reads && (!d_q || !d_off || !d_len || !d_out))) return fail(DTWAX_EINVAL, "NULL argument");
   if (!p->ctx->n_cols) return fail(DTWAX_ENOTREADY, "no reference uploaded");
-  return run_plan(p, d_q, d_off, d_len, d_out, S_(st), nullptr, nullptr, 0, nullptr, 0);
+  return run_plan(p, d_q, d_off, d_len, d_out, S_(st));
 }
 
 int dtwax_score(dtwax_ctx* c, const float* d_q, const int64_t* d_off, const int32_t* d_len, int32_t n_reads,
@@ -543,7 +547,7 @@ int dtwax_distances(dtwax_ctx* c, const void* ref, const void* query, void* dist
     c->compat_plan_n = num_entries;
   }
   int rc = run_plan(c->compat_plan, c->compat_q, c->compat_off, c->compat_len,
-                    reinterpret_cast<dtwax_result*>(c->compat_res), st, nullptr, nullptr, 0, nullptr, 0);
+                    reinterpret_cast<dtwax_result*>(c->compat_res), st);
   if (rc) return rc;
   HIP_TRY(launch_results_to_half2(c->compat_res, num_entries, (uint32_t*)dists, st));
   return DTWAX_OK;

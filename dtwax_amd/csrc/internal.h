@@ -1,0 +1,73 @@
+// internal.h -- host-side structures shared by the C ABI implementation files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "dtwax.h"
+#include "kernels.h"
+
+namespace dtwax {
+
+int set_error(int code, const std::string& msg);
+
+#define DTWAX_HIP_TRY(expr)                                                                     \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return ::dtwax::set_error(DTWAX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// One scoring launch: reset keys and counter, persistent sDTW kernel over
+// `n_items` items, finalise `n_reads` results.  ev0/ev1 (optional) bracket
+// the sDTW kernel on `st`.
+struct ScoreLaunch {
+  int precision = DTWAX_F16;
+  int S = 32;
+  int nstr = 2;
+  int grid = 1;
+  const float* queries = nullptr;
+  const int64_t* q_off = nullptr;
+  const int32_t* q_len = nullptr;
+  const WorkItem* items = nullptr;
+  int32_t n_items = 0;
+  int32_t n_reads = 0;
+  int32_t* counter = nullptr;
+  unsigned long long* keys = nullptr;
+  void* scratch = nullptr;
+  int64_t scratch_stride = 0;
+  void* state = nullptr;
+  int64_t row_stride = 0;
+  float renorm_k = 0.f;
+  DeviceResult* out = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+}  // namespace dtwax
+
+struct dtwax_ctx {
+  int device = 0;
+  int n_cu = 0;
+  int clock_khz = 0;
+  std::string name;
+  // reference
+  int32_t n_strands = 0;
+  int64_t n_cols = 0;
+  void* ref16_alloc = nullptr;  // h2 per column, COL_PADL + n + COL_PADR
+  void* ref32_alloc = nullptr;  // n_strands floats per column, same padding
+  // compat entry point buffers
+  void* compat_ref = nullptr;
+  float* compat_q = nullptr;
+  int64_t* compat_off = nullptr;
+  int32_t* compat_len = nullptr;
+  dtwax::DeviceResult* compat_res = nullptr;
+  int32_t compat_cap = 0;
+  struct dtwax_plan* compat_plan = nullptr;
+  int32_t compat_plan_n = -1;
+};
+
+namespace dtwax {
+int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st);
+int occupancy(int precision, int S, int nstr, bool inj);
+}  // namespace dtwax

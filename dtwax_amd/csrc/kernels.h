@@ -21,12 +21,19 @@ constexpr int SCRATCH_PADL = COL_PADL;
 // the overlap that makes a chunk exact under the no-deletion recurrence).
 struct WorkItem {
   int32_t read;
-  int32_t slot;  // Read Until state slot (row_in / row_out), else 0
+  int32_t slot;  // Read Until state slot, else 0
   int32_t x;
   int32_t c0;
   int32_t c1;
-  int32_t reserved[3];
+  int32_t flags;  // ITEM_* below (Read Until only)
+  int32_t cur;    // Read Until: which of the slot's two rows holds the current last row
+  int32_t reserved;
 };
+
+// Read Until item flags: resume from the slot's stored row (else free start);
+// store the new last row into the slot's other row.
+constexpr int32_t ITEM_RESUME = 1;
+constexpr int32_t ITEM_STORE = 2;
 
 struct DeviceResult {
   float score[2];
@@ -45,9 +52,9 @@ struct KParams {
   unsigned long long* keys;  // 2 per read: (ordered score bits << 32) | column
   void* scratch;             // per workgroup band rows
   int64_t scratch_stride;    // columns per workgroup (includes the pads)
-  const void* row_in;        // Read Until: column 0 of slot 0's input row
-  void* row_out;             // Read Until: column 0 of slot 0's output row
-  int64_t row_stride;        // columns per slot
+  void* state;               // Read Until: column 0 of slot 0's row 0; slot s row k
+                             // starts (2*s + k) * row_stride columns further
+  int64_t row_stride;        // columns per state row (includes the pads)
   float renorm_k;            // F16_RENORM per-lane drift constant (S rows' worth)
 };
 
@@ -63,6 +70,15 @@ int occupancy_f32(int S, int nstr, bool inj);
 // normalisation (normalize_kernels.hip)
 hipError_t launch_normalize(const float* raw, const int64_t* off, const int32_t* len, int32_t n_reads,
                             int mode, double eps, float* out, double* stats, hipStream_t st);
+
+// Read Until chunk normalisation: first chunks fix (centre, inv) per slot,
+// later chunks apply the slot's values.
+hipError_t launch_ru_normalize(const float* raw, int32_t n_chunks, int32_t chunk_len, const int32_t* slot,
+                               const int32_t* first, int mode, double eps, double* slot_stats,
+                               float* out, hipStream_t st);
+
+// fill n 32-bit words with `value` (a kernel, so it may be captured in a graph)
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t st);
 
 // reference-compat helpers (compat_kernels.hip)
 hipError_t launch_untile_reference(const uint32_t* tiled, int32_t ref_len, int32_t seg, int32_t warp,

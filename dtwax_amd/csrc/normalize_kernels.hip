@@ -85,21 +85,14 @@ __device__ uint64_t radix_select(int32_t n, int64_t k, KeyF keyf, uint32_t* hist
   return prefix;
 }
 
-__global__ __launch_bounds__(NT) void normalize_kernel(const float* __restrict__ raw,
-                                                       const int64_t* __restrict__ off,
-                                                       const int32_t* __restrict__ len,
-                                                       int mode, double eps,
-                                                       float* __restrict__ out,
-                                                       double* __restrict__ stats) {
-  __shared__ double sh[NT / 64 + 1];
-  __shared__ uint32_t hist[256];
-  __shared__ uint64_t shv[2];
-  const int32_t r = blockIdx.x;
-  const int32_t n = len[r];
-  const float* x = raw + off[r];
-  float* y = out + off[r];
-  if (n <= 0) return;
-  double centre = 0.0, scale = 1.0;
+// Statistics of one read: centre, scale and the multiplier inv applied as
+// y = (float)((x - centre) * inv).  NONE gives centre 0, inv 1 (a copy).
+__device__ void read_stats(const float* __restrict__ x, int32_t n, int mode, double eps, double* sh,
+                           uint32_t* hist, uint64_t* shv, double& centre, double& scale,
+                           double& inv) {
+  centre = 0.0;
+  scale = 1.0;
+  inv = 1.0;
   if (mode == DTWAX_NORM_MEANSTD) {
     double s = 0.0;
     for (int32_t i = threadIdx.x; i < n; i += NT) s += (double)x[i];
@@ -112,8 +105,7 @@ __global__ __launch_bounds__(NT) void normalize_kernel(const float* __restrict__
     const double var = block_sum(v, sh) / (double)n;
     centre = mean;
     scale = sqrt(var + eps);
-    const double inv = 1.0 / scale;
-    for (int32_t i = threadIdx.x; i < n; i += NT) y[i] = (float)(((double)x[i] - mean) * inv);
+    inv = 1.0 / scale;
   } else if (mode == DTWAX_NORM_MEDMAD) {
     auto kf = [&](int32_t i) -> uint64_t { return fkey(x[i]); };
     double med;
@@ -135,10 +127,30 @@ __global__ __launch_bounds__(NT) void normalize_kernel(const float* __restrict__
     }
     centre = med;
     scale = 1.4826 * mad;
-    const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
-    for (int32_t i = threadIdx.x; i < n; i += NT) y[i] = (float)(((double)x[i] - med) * inv);
-  } else {
+    inv = scale > 0.0 ? 1.0 / scale : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(NT) void normalize_kernel(const float* __restrict__ raw,
+                                                       const int64_t* __restrict__ off,
+                                                       const int32_t* __restrict__ len,
+                                                       int mode, double eps,
+                                                       float* __restrict__ out,
+                                                       double* __restrict__ stats) {
+  __shared__ double sh[NT / 64 + 1];
+  __shared__ uint32_t hist[256];
+  __shared__ uint64_t shv[2];
+  const int32_t r = blockIdx.x;
+  const int32_t n = len[r];
+  const float* x = raw + off[r];
+  float* y = out + off[r];
+  if (n <= 0) return;
+  double centre, scale, inv;
+  read_stats(x, n, mode, eps, sh, hist, shv, centre, scale, inv);
+  if (mode == DTWAX_NORM_NONE) {
     for (int32_t i = threadIdx.x; i < n; i += NT) y[i] = x[i];
+  } else {
+    for (int32_t i = threadIdx.x; i < n; i += NT) y[i] = (float)(((double)x[i] - centre) * inv);
   }
   if (stats && threadIdx.x == 0) {
     stats[2 * (int64_t)r] = centre;
@@ -146,11 +158,58 @@ __global__ __launch_bounds__(NT) void normalize_kernel(const float* __restrict__
   }
 }
 
+// Read Until chunks: chunk i (chunk_len samples at raw + i * chunk_len) of
+// the read in slot[i].  A read's first chunk (first[i] != 0) fixes the read's
+// statistics (stored as centre, inv in slot_stats[slot]); later chunks apply
+// them, so the chunks of a read are normalised as one query would be with
+// the first chunk's statistics.
+__global__ __launch_bounds__(NT) void ru_normalize_kernel(const float* __restrict__ raw,
+                                                          int32_t chunk_len,
+                                                          const int32_t* __restrict__ slot,
+                                                          const int32_t* __restrict__ first,
+                                                          int mode, double eps,
+                                                          double* __restrict__ slot_stats,
+                                                          float* __restrict__ out) {
+  __shared__ double sh[NT / 64 + 1];
+  __shared__ uint32_t hist[256];
+  __shared__ uint64_t shv[2];
+  __shared__ double cs[2];
+  const int32_t c = blockIdx.x;
+  const float* x = raw + (int64_t)c * chunk_len;
+  float* y = out + (int64_t)c * chunk_len;
+  double* st = slot_stats + 2 * (int64_t)slot[c];
+  if (first[c]) {
+    double centre, scale, inv;
+    read_stats(x, chunk_len, mode, eps, sh, hist, shv, centre, scale, inv);
+    if (threadIdx.x == 0) {
+      st[0] = centre;
+      st[1] = inv;
+      cs[0] = centre;
+      cs[1] = inv;
+    }
+  } else if (threadIdx.x == 0) {
+    cs[0] = st[0];
+    cs[1] = st[1];
+  }
+  __syncthreads();
+  const double centre = cs[0], inv = cs[1];
+  for (int32_t i = threadIdx.x; i < chunk_len; i += NT) y[i] = (float)(((double)x[i] - centre) * inv);
+}
+
 hipError_t launch_normalize(const float* raw, const int64_t* off, const int32_t* len, int32_t n_reads,
                             int mode, double eps, float* out, double* stats, hipStream_t st) {
   if (n_reads <= 0) return hipSuccess;
   hipLaunchKernelGGL(normalize_kernel, dim3(n_reads), dim3(NT), 0, st, raw, off, len, mode, eps, out,
                      stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_ru_normalize(const float* raw, int32_t n_chunks, int32_t chunk_len, const int32_t* slot,
+                               const int32_t* first, int mode, double eps, double* slot_stats,
+                               float* out, hipStream_t st) {
+  if (n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ru_normalize_kernel, dim3(n_chunks), dim3(NT), 0, st, raw, chunk_len, slot, first,
+                     mode, eps, slot_stats, out);
   return hipGetLastError();
 }
 

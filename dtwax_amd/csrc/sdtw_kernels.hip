@@ -149,6 +149,8 @@ struct SweepIO {
   int32_t x, c0, c1;
   int32_t inj_lane;  // lane that takes the top row (first real lane)
   bool final_band;
+  bool top_diag;     // lane 0 takes trow[x-1] as the diagonal of its first cell (resumed
+                     // top row; needs no virtual rows in front of the band)
 };
 
 // Sweep one band across columns [x, c1) (+ ramp).  Accumulates the min/argmin
@@ -187,6 +189,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
   for (int i = 0; i < S; i++) D0[i] = INF;
   h2 tin = INF, dg = INF, hist = INF;
   if (lane == 0) tin = ring_t[64] - K;  // top row at column x for lane 0
+  if (lane == 0 && io.top_diag) dg = ring_t[63] - K;
   if (INJ && lane == io.inj_lane) tin = ring_t[64 - lane] - K;
 
   for (int32_t T = 0; T < nsteps; T += 64) {
@@ -232,7 +235,9 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
       }
     }
     if (MODE == DTWAX_F16_RENORM && !io.final_band && in_item) rowmin = hmin2(rowmin, hist);
-    if (io.orow && in_item) io.orow[col] = hist;
+    // band scratch rows keep the whole item; the final row only [c0, c1), so
+    // the items of one read write disjoint columns of a stored row
+    if (io.orow && in_item && (!io.final_band || col >= io.c0)) io.orow[col] = hist;
     // Ring refill for columns x + T + 128 + lane.
     __builtin_amdgcn_wave_barrier();
     const int slot = (T + 192 + lane) & (RING - 1);
@@ -258,8 +263,11 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
 
   const h2* trow_ext = nullptr;
   h2* orow_ext = nullptr;
-  if (p.row_in) trow_ext = (const h2*)p.row_in + (int64_t)it.slot * p.row_stride;
-  if (p.row_out) orow_ext = (h2*)p.row_out + (int64_t)it.slot * p.row_stride;
+  if (p.state) {
+    h2* base = (h2*)p.state + (int64_t)it.slot * 2 * p.row_stride;
+    if (it.flags & ITEM_RESUME) trow_ext = base + (int64_t)it.cur * p.row_stride;
+    if (it.flags & ITEM_STORE) orow_ext = base + (int64_t)(1 - it.cur) * p.row_stride;
+  }
 
   h2 best = hsplat((half_t)__builtin_inff());
   int32_t posf = -1, posr = -1;
@@ -298,6 +306,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     io.c0 = it.c0;
     io.c1 = it.c1;
     io.inj_lane = (b == 0) ? (P / S) : 0;
+    io.top_diag = (b == 0) && (trow_ext != nullptr);
     h2 rowmin = hsplat((half_t)__builtin_inff());
     h2 Kl = Kb;
     if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = Kb + prev_min;
@@ -398,8 +407,11 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
   const float* __restrict__ ref = (const float*)p.ref;  // NSTR floats per column
   const float* trow_ext = nullptr;
   float* orow_ext = nullptr;
-  if (p.row_in) trow_ext = (const float*)p.row_in + (int64_t)it.slot * p.row_stride * NSTR;
-  if (p.row_out) orow_ext = (float*)p.row_out + (int64_t)it.slot * p.row_stride * NSTR;
+  if (p.state) {
+    float* base = (float*)p.state + (int64_t)it.slot * 2 * p.row_stride * NSTR;
+    if (it.flags & ITEM_RESUME) trow_ext = base + (int64_t)it.cur * p.row_stride * NSTR;
+    if (it.flags & ITEM_STORE) orow_ext = base + (int64_t)(1 - it.cur) * p.row_stride * NSTR;
+  }
 
   float best[NSTR];
   int32_t pos[NSTR];
@@ -450,6 +462,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
       dg[st] = INF;
       hist[st] = INF;
       if (lane == 0) tin[st] = ring_t[64 * NSTR + st];
+      if (lane == 0 && b == 0 && trow_ext) dg[st] = ring_t[63 * NSTR + st];
       if (INJ && lane == inj) tin[st] = ring_t[(64 - lane) * NSTR + st];
     }
 
@@ -479,7 +492,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
           best[st] = hist[st];
           pos[st] = col;
         }
-        if (orow && in_item) orow[(int64_t)col * NSTR + st] = hist[st];
+        if (orow && in_item && (!final_band || col >= c0)) orow[(int64_t)col * NSTR + st] = hist[st];
       }
       __builtin_amdgcn_wave_barrier();
       const int slot = (T + 192 + lane) & (RING - 1);
@@ -545,6 +558,12 @@ __global__ void finalize_kernel(const unsigned long long* keys, int32_t n_reads,
     }
   }
   out[i] = r;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t value) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = value;
 }
 
 __global__ void selftest_lanes_kernel(int32_t* out) {
@@ -623,6 +642,13 @@ hipError_t launch_finalize(const unsigned long long* keys, int32_t n_reads, int3
 
 hipError_t launch_selftest_lanes(int32_t* d_out, hipStream_t st) {
   hipLaunchKernelGGL(selftest_lanes_kernel, dim3(1), dim3(64), 0, st, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, value);
   return hipGetLastError();
 }
 

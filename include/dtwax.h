@@ -176,6 +176,89 @@ int dtwax_distances(dtwax_ctx* ctx, const void* ref, const void* query, void* di
                     int32_t num_entries, uint32_t thresh, void* stream,
                     void* device_last_row, int32_t compat_quirks);
 
+/* ---- Read Until serving ---------------------------------------------------
+ * Replaces the stateless loop of src/ru_main.cu:172-326 (10 FIFO queries of
+ * 2048 samples scored from scratch per iteration, scores printed, no
+ * decision).  A read arrives as fixed-size chunks of raw samples on a
+ * channel.  Per undecided read the engine keeps the last DP row over the
+ * whole reference (both strands) in HBM, so chunk k resumes the alignment
+ * of chunks 0..k-1: the score after k chunks equals the score of the
+ * concatenated query, normalised with the first chunk's statistics, scored
+ * from scratch with the same precision.  After each chunk the running score
+ * s = min(fwd, rev) decides: s >= threshold is NONTARGET at once (the min
+ * over the last row never decreases as rows are added, so the decision is
+ * final); a read still below it after max_chunks chunks is TARGET.
+ * Decision rule as calc_classification_threshold.py:49-57 (target iff
+ * score < threshold). */
+typedef struct dtwax_ru dtwax_ru;
+
+typedef enum {
+  DTWAX_RU_CONTINUE = 0,  /* undecided: send the next chunk */
+  DTWAX_RU_NONTARGET = 1, /* score >= threshold: unblock the read */
+  DTWAX_RU_TARGET = 2,    /* max_chunks chunks below the threshold: keep it */
+  DTWAX_RU_NO_SLOT = 3,   /* new read found every state slot busy: not scored, and
+                             later chunks of it are STALE */
+  DTWAX_RU_STALE = 4      /* chunk of a read already decided: not scored; score and
+                             chunks are the decided read's */
+} dtwax_ru_action;
+
+typedef struct {
+  int32_t max_reads;     /* state slots = reads undecided at once (0: 512) */
+  int32_t chunk_len;     /* samples per chunk, a multiple of 64 (0: 1600) */
+  int32_t max_chunks;    /* chunks before a TARGET decision (0: 4) */
+  int32_t precision;     /* DTWAX_F16 or DTWAX_F32 */
+  int32_t segment;       /* rows per lane, dividing chunk_len / 64 (0: largest of 32, 25, 16, 8) */
+  int32_t norm;          /* dtwax_norm_mode, statistics of the read's first chunk */
+  double eps;            /* DTWAX_NORM_MEANSTD epsilon */
+  float threshold;       /* on min(fwd, rev); <= 0: no early NONTARGET */
+  int32_t min_ref_chunk; /* shortest reference piece per work item, columns (0: 4 * chunk_len) */
+  int64_t hbm_budget;    /* bytes of read state at most (0: max_reads slots) */
+  int32_t reserved[4];
+} dtwax_ru_opts;
+
+typedef struct {
+  int32_t channel; /* >= 0 */
+  int32_t number;  /* read number on the channel: a new number ends the previous read */
+  int64_t offset;  /* first sample of this chunk in the samples buffer */
+} dtwax_ru_chunk;
+
+typedef struct {
+  int32_t channel;
+  int32_t number;
+  int32_t action;     /* dtwax_ru_action */
+  int32_t chunks;     /* chunks of the read scored so far */
+  float score[2];     /* running fwd, rev score (+inf if never scored) */
+  int32_t argmin[2];  /* column of each */
+  double latency_us;  /* serving loop only: submission to decision available */
+} dtwax_ru_decision;
+
+/* Allocates max_reads (or what hbm_budget allows) state slots of
+ * 2 x (n_cols + 448) x 4 bytes (x n_strands for DTWAX_F32) plus staging, on
+ * the context's current reference (which must not change while the engine
+ * lives). */
+int dtwax_ru_create(dtwax_ctx* ctx, const dtwax_ru_opts* opts, dtwax_ru** out);
+int dtwax_ru_destroy(dtwax_ru* ru);
+int dtwax_ru_info(const dtwax_ru* ru, int32_t* slots, int32_t* active, int32_t* segment,
+                  int64_t* state_bytes);
+int dtwax_ru_stats(const dtwax_ru* ru, int64_t* batches, int64_t* chunks, int64_t* cells,
+                   double* gpu_ms);
+/* Scores one batch: chunk i is chunk_len host samples at samples + offset.
+ * At most one chunk per channel per call (EINVAL otherwise).  Writes one
+ * decision per chunk into out[i].  Synchronous; calls are serialised. */
+int dtwax_ru_process(dtwax_ru* ru, const dtwax_ru_chunk* chunks, int32_t n, const float* samples,
+                     dtwax_ru_decision* out);
+/* The read on `channel` ended: free its state. */
+int dtwax_ru_release(dtwax_ru* ru, int32_t channel);
+/* Serving loop: a worker thread scores, as one batch, every chunk submitted
+ * while the previous batch ran (the earliest chunk per channel; later ones
+ * wait for the next batch).  submit copies chunk_len samples; poll returns up
+ * to cap decisions, waiting up to timeout_ms (-1: until one is available). */
+int dtwax_ru_start(dtwax_ru* ru);
+int dtwax_ru_submit(dtwax_ru* ru, int32_t channel, int32_t number, const float* samples);
+int dtwax_ru_poll(dtwax_ru* ru, dtwax_ru_decision* out, int32_t cap, int32_t* n_out,
+                  int32_t timeout_ms);
+int dtwax_ru_stop(dtwax_ru* ru);
+
 /* ---- self-test of the lane-movement primitives the kernels rely on ------
  * Writes, for each of the 64 lanes, the value received by a DPP wave shift
  * right by one (out[0..63]) and left by one (out[64..127]) of the lane

@@ -1,0 +1,206 @@
+"""GPU parity of the Read Until serving engine (dtwax_ru_* through the C ABI).
+
+The engine keeps each undecided read's last DP row in HBM and extends it
+chunk by chunk.  Bar: after every chunk, the running (fwd, rev) scores and
+argmins are bit-identical to
+  * the oracle extended chunk by chunk from its own stored row
+    (orc_sdtw_* with init_row), and
+  * the batch path scoring the concatenated chunks from scratch,
+with the chunks normalised by the first chunk's median/MAD statistics.
+Chunks of different reads share batches in staggered order, so each check
+also covers "a read scores the same alone and in a mixed batch".
+"""
+import numpy as np
+import pytest
+
+import dtwax_amd as D
+import oracle as O
+from dtwax_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = D.Context(0)
+    yield c
+    c.close()
+
+
+def _workload(seed, n_reads, chunk_len, n_chunks, ref_bp=8_000):
+    w = synth.config_workload(seed=seed, ref_bp=ref_bp, n_reads=n_reads,
+                              raw_len=chunk_len * n_chunks, target_frac=0.5)
+    return w
+
+
+def _norm_chunks(raw, chunk_len):
+    """Median/MAD statistics of the first chunk applied to every chunk, as
+    y = (float)((x - centre) * inv) in binary64."""
+    first = raw[:chunk_len]
+    y0, med, mad = O.norm_medmad(first)
+    scale = 1.4826 * mad
+    inv = 1.0 / scale if scale > 0 else 0.0
+    y = ((raw.astype(np.float64) - med) * inv).astype(np.float32)
+    assert np.array_equal(y[:chunk_len], y0)
+    return y
+
+
+def _oracle_chunked(y, ref, chunk_len, precision):
+    """Scores/argmins after each chunk, extending the oracle's stored row."""
+    n_chunks = len(y) // chunk_len
+    out = np.zeros((n_chunks, 2), np.float32)
+    arg = np.zeros((n_chunks, 2), np.int32)
+    for st in range(ref.shape[0]):
+        row = None
+        for k in range(n_chunks):
+            q = y[k * chunk_len:(k + 1) * chunk_len]
+            if precision == "f16":
+                sc, am, row = O.sdtw_f16(O.f32_to_f16(q), O.f32_to_f16(ref[st]), init_row=row,
+                                         want_row=True)
+                out[k, st] = O.f16_to_f32(np.array([sc], np.uint16))[0]
+            else:
+                sc, am, row = O.sdtw_f32(q, ref[st], init_row=row, want_row=True)
+                out[k, st] = sc
+            arg[k, st] = am
+    return out, arg
+
+
+def _run_staggered(ru, reads, chunk_len, n_chunks, stagger=3):
+    """Read i sends chunk k in batch (i % stagger) + k; returns per read the
+    list of decisions in chunk order."""
+    n = len(reads)
+    got = [[] for _ in range(n)]
+    n_batches = stagger + n_chunks
+    for b in range(n_batches):
+        chans, nums, rows = [], [], []
+        for i in range(n):
+            k = b - (i % stagger)
+            if 0 <= k < n_chunks:
+                chans.append(i)
+                nums.append(1)
+                rows.append(reads[i][k * chunk_len:(k + 1) * chunk_len])
+        if not rows:
+            continue
+        dec = ru.process(chans, nums, np.stack(rows))
+        for d in dec:
+            got[d["channel"]].append(d)
+    return got
+
+
+@pytest.mark.parametrize("precision,chunk_len,min_ref_chunk", [
+    ("f16", 1600, 0),      # S=25, one band, whole-reference pieces at this size
+    ("f16", 1600, 2048),   # many reference pieces: resumed rows across piece overlaps
+    ("f16", 1024, 1024),   # S=16
+    ("f16", 3200, 4096),   # two bands per chunk (band scratch between them)
+    ("f32", 1600, 2048),
+])
+def test_ru_chunked_matches_oracle_and_scratch(ctx, precision, chunk_len, min_ref_chunk):
+    n_chunks = 4
+    w = _workload(seed=chunk_len + min_ref_chunk, n_reads=6, chunk_len=chunk_len,
+                  n_chunks=n_chunks)
+    ref = w.ref_levels.astype(np.float32)
+    ctx.upload_reference(ref)
+    ru = D.ReadUntil(ctx, max_reads=8, chunk_len=chunk_len, max_chunks=n_chunks,
+                     precision=precision, min_ref_chunk=min_ref_chunk, threshold=0.0)
+    try:
+        got = _run_staggered(ru, w.reads, chunk_len, n_chunks)
+    finally:
+        ru.close()
+    for i, raw in enumerate(w.reads):
+        y = _norm_chunks(raw, chunk_len)
+        osc, oam = _oracle_chunked(y, ref, chunk_len, precision)
+        assert len(got[i]) == n_chunks
+        for k, d in enumerate(got[i]):
+            assert d["chunks"] == k + 1
+            assert d["action"] == (D.RU_TARGET if k == n_chunks - 1 else D.RU_CONTINUE)
+            assert np.array_equal(d["score"], osc[k]), (i, k, d["score"], osc[k])
+            assert np.array_equal(d["argmin"], oam[k]), (i, k, d["argmin"], oam[k])
+        # the batch path from scratch on the concatenation, after every chunk
+        sc, am = ctx.score([y[:(k + 1) * chunk_len] for k in range(n_chunks)],
+                           precision=precision)
+        assert np.array_equal(sc, osc)
+        assert np.array_equal(am, oam)
+
+
+def test_ru_decisions_slots_and_staleness(ctx):
+    L, n_chunks = 1600, 4
+    w = _workload(seed=5, n_reads=6, chunk_len=L, n_chunks=n_chunks)
+    ref = w.ref_levels.astype(np.float32)
+    ctx.upload_reference(ref)
+    ys = [_norm_chunks(r, L) for r in w.reads]
+    osc = [_oracle_chunked(y, ref, L, "f16")[0].min(axis=1) for y in ys]
+    # threshold between the targets' and the non-targets' score after chunk 1
+    t_first = np.array([s[0] for s in osc])
+    thr = float(0.5 * (t_first[w.is_target].max() + t_first[~w.is_target].min()))
+    assert t_first[w.is_target].max() < thr <= t_first[~w.is_target].min()
+    ru = D.ReadUntil(ctx, max_reads=3, chunk_len=L, max_chunks=n_chunks, precision="f16",
+                     threshold=thr)
+    try:
+        assert ru.info()["slots"] == 3
+        chans = list(range(6))
+        dec = ru.process(chans, [1] * 6, np.stack([r[:L] for r in w.reads]))
+        # the first three reads take the three slots, the rest find none
+        for i, d in enumerate(dec):
+            if i >= 3:
+                assert d["action"] == D.RU_NO_SLOT
+                continue
+            exp = D.RU_NONTARGET if osc[i][0] >= thr else D.RU_CONTINUE
+            assert d["action"] == exp
+            assert d["score"].min() == osc[i][0]
+        # decided reads freed their slots; a NO_SLOT read stays unscored
+        dec2 = ru.process([3], [1], w.reads[3][L:2 * L][None])
+        assert dec2[0]["action"] == D.RU_STALE
+        # a new read number on channel 3 is a new read and can take a freed slot
+        n_free = 3 - ru.info()["active"]
+        assert n_free == int(sum(d["action"] == D.RU_NONTARGET for d in dec[:3])) >= 1
+        dec3 = ru.process([3], [2], w.reads[3][:L][None])
+        assert dec3[0]["action"] in (D.RU_CONTINUE, D.RU_NONTARGET)
+        assert dec3[0]["score"].min() == osc[3][0]
+        # chunks of decided reads are stale and carry the decided score
+        for i in range(3):
+            if dec[i]["action"] == D.RU_NONTARGET:
+                d = ru.process([i], [1], w.reads[i][L:2 * L][None])[0]
+                assert d["action"] == D.RU_STALE and d["chunks"] == 1
+        # release frees the slot of an undecided read
+        before = ru.info()["active"]
+        for i in range(3):
+            if dec[i]["action"] == D.RU_CONTINUE:
+                ru.release(i)
+                assert ru.info()["active"] == before - 1
+                break
+        with pytest.raises(D.DtwaxError):
+            ru.process([0, 0], [1, 1], np.zeros((2, L), np.float32))
+    finally:
+        ru.close()
+
+
+def test_ru_serving_loop_matches_process(ctx):
+    L, n_chunks = 1600, 3
+    w = _workload(seed=9, n_reads=5, chunk_len=L, n_chunks=n_chunks)
+    ref = w.ref_levels.astype(np.float32)
+    ctx.upload_reference(ref)
+    ru = D.ReadUntil(ctx, max_reads=8, chunk_len=L, max_chunks=n_chunks, precision="f16")
+    try:
+        ru.start()
+        # every chunk of every read up front: the loop must keep per-channel order
+        for k in range(n_chunks):
+            for i, r in enumerate(w.reads):
+                ru.submit(i, 7, r[k * L:(k + 1) * L])
+        got = []
+        while len(got) < n_chunks * len(w.reads):
+            d = ru.poll(64, timeout_ms=20000)
+            assert len(d), "serving loop produced no decision within 20 s"
+            got.extend(d)
+        ru.stop()
+        st = ru.stats()
+        assert st["chunks"] == n_chunks * len(w.reads)
+    finally:
+        ru.close()
+    for i, raw in enumerate(w.reads):
+        y = _norm_chunks(raw, L)
+        osc, oam = _oracle_chunked(y, ref, L, "f16")
+        mine = [d for d in got if d["channel"] == i]
+        assert [int(d["chunks"]) for d in mine] == list(range(1, n_chunks + 1))
+        for k, d in enumerate(mine):
+            assert np.array_equal(d["score"], osc[k])
+            assert d["latency_us"] > 0
