@@ -282,8 +282,11 @@ int process_locked(dtwax_ru* r, const dtwax_ru_chunk* chunks, int32_t n, const f
       rec.argmin[k] = (k < r->nstr) ? res.argmin[k] : -1;
     }
     const float best = std::min(rec.score[0], rec.score[1]);
+    const float limit = r->o.threshold_per_row > 0.0f
+                            ? r->o.threshold_per_row * (float)((int64_t)rec.chunks * L)
+                            : r->o.threshold;
     int32_t action = DTWAX_RU_CONTINUE;
-    if (best >= r->o.threshold) action = DTWAX_RU_NONTARGET;
+    if (best >= limit) action = DTWAX_RU_NONTARGET;
     else if (rec.chunks >= r->o.max_chunks) action = DTWAX_RU_TARGET;
     if (action == DTWAX_RU_CONTINUE) {
       r->slot_cur[(size_t)rec.slot] ^= 1;  // the new row is in the other half

@@ -186,8 +186,10 @@ int dtwax_distances(dtwax_ctx* ctx, const void* ref, const void* query, void* di
  * concatenated query, normalised with the first chunk's statistics, scored
  * from scratch with the same precision.  After each chunk the running score
  * s = min(fwd, rev) decides: s >= threshold is NONTARGET at once (the min
- * over the last row never decreases as rows are added, so the decision is
- * final); a read still below it after max_chunks chunks is TARGET.
+ * over the last row never decreases as rows are added, so under a fixed
+ * threshold the decision is final); a read still below it after max_chunks
+ * chunks is TARGET.  The length-aware rule (threshold_per_row) compares s
+ * with a threshold proportional to the rows scored so far instead.
  * Decision rule as calc_classification_threshold.py:49-57 (target iff
  * score < threshold). */
 typedef struct dtwax_ru dtwax_ru;
@@ -213,7 +215,9 @@ typedef struct {
   float threshold;       /* on min(fwd, rev); <= 0: no early NONTARGET */
   int32_t min_ref_chunk; /* shortest reference piece per work item, columns (0: 4 * chunk_len) */
   int64_t hbm_budget;    /* bytes of read state at most (0: max_reads slots) */
-  int32_t reserved[4];
+  float threshold_per_row; /* > 0: length-aware rule, the threshold after k chunks is
+                              threshold_per_row * k * chunk_len (replaces threshold) */
+  int32_t reserved[3];
 } dtwax_ru_opts;
 
 typedef struct {

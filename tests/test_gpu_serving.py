@@ -204,3 +204,30 @@ def test_ru_serving_loop_matches_process(ctx):
         for k, d in enumerate(mine):
             assert np.array_equal(d["score"], osc[k])
             assert d["latency_us"] > 0
+
+
+def test_ru_simulated_flow_cell(ctx):
+    """64 channels for 6 simulated seconds against a 20 kb reference: every
+    read that completes its chunks is decided, decisions are accurate, and
+    the engine's chunk count matches the simulator's."""
+    from dtwax_amd import ru_sim
+    rng = np.random.default_rng(21)
+    means, _ = synth.make_model(rng)
+    seq, levels = synth.build_reference(rng, 20_000, means)
+    ctx.upload_reference(levels)
+    src = ru_sim.ReadSource(seq, means, np.random.default_rng(22))
+    thr, cal_acc = ru_sim.calibrate_threshold_per_row(ctx, src, 1600, 4, n_each=16)
+    assert cal_acc > 0.9
+    ru = D.ReadUntil(ctx, max_reads=64, chunk_len=1600, max_chunks=4, precision="f16",
+                     threshold_per_row=thr)
+    try:
+        cfg = ru_sim.SimConfig(n_channels=64, duration_s=6.0, target_frac=0.3, seed=3)
+        m = ru_sim.simulate(ru, src, cfg)
+        st = ru.stats()
+    finally:
+        ru.close()
+    print(m)
+    assert m["reads_decided"] > 50
+    assert m["accuracy"] > 0.9
+    assert m["chunks_scored"] == st["chunks"]
+    assert 0 < m["latency_ms_p50"] <= m["latency_ms_p99"]
