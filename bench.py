@@ -63,23 +63,29 @@ def measured_traffic():
         return None
 
 
-def read_until_leg(ctx, seconds):
+def read_until_leg(ctx, seconds, norm="meanstd"):
     """Read Until serving on the 5 Mb workload: 512 channels, 1600-sample
     chunks (0.4 s at 4 kHz), decisions within 4 chunks, FP16, median/MAD of
-    the first chunk.  (1) a saturated batch: one first chunk on every channel
-    at once; (2) a simulated flow cell replayed through the engine, with the
-    per-row threshold calibrated on 64 reads by the engine itself."""
+    the first chunk ("meanstd": the reference's own mean/std on both the
+    reference squiggle and the reads; "medmad": median/MAD on both).
+    (1) a saturated batch: one first chunk on every channel
+    at once; (2) a simulated flow cell driven in wall-clock time through the
+    engine's serving loop (continuous batching on 2 streams), with the
+    per-row threshold calibrated on 64 reads by the engine itself; (3) the
+    same flow cell served by the plain batched form (each read classified
+    once from scratch after its 4 chunks), threshold calibrated the same way."""
     import dtwax_amd as D
     from dtwax_amd import ru_sim, synth
     t0 = time.time()
     rng = np.random.default_rng(5)
     means, _ = synth.make_model(rng)
-    seq, levels = synth.build_reference(rng, 5_000_000, means)
+    seq, levels = synth.build_reference(rng, 5_000_000, means, ref_norm=norm)
     ctx.upload_reference(levels)
+    mode = D.NORM_MEANSTD if norm == "meanstd" else D.NORM_MEDMAD
     src = ru_sim.ReadSource(seq, means, np.random.default_rng(6))
-    thr, cal_acc = ru_sim.calibrate_threshold_per_row(ctx, src, 1600, 4, n_each=32)
+    thr, cal_acc = ru_sim.calibrate_threshold_per_row(ctx, src, 1600, 4, n_each=128, norm=mode)
     ru = D.ReadUntil(ctx, max_reads=512, chunk_len=1600, max_chunks=4, precision="f16",
-                     threshold_per_row=thr)
+                     threshold_per_row=thr, norm=mode)
     try:
         info = ru.info()
         sig = np.stack([src.signal(1600, i % 10 == 0) for i in range(512)])
@@ -96,13 +102,17 @@ def read_until_leg(ctx, seconds):
         wall, kms, cells = min(sat)
         print(f"[bench] read until: saturated batch {wall * 1e3:.1f} ms", file=sys.stderr, flush=True)
         cfg = ru_sim.SimConfig(n_channels=512, duration_s=seconds, seed=1)
-        m = ru_sim.simulate(ru, src, cfg)
-        print(f"[bench] read until: simulated {m['sim_seconds']} s", file=sys.stderr, flush=True)
+        m = ru_sim.simulate_realtime(ru, src, cfg)
+        print(f"[bench] read until: served {m['sim_seconds']} s", file=sys.stderr, flush=True)
     finally:
         ru.close()
+    pthr, pcal = ru_sim.calibrate_plain_threshold(ctx, src, 4 * 1600, n_each=128, norm=mode)
+    plain = ru_sim.simulate_plain(ctx, src, cfg, pthr, norm=mode)
+    plain.update({"threshold_per_row": pthr, "calibration_accuracy": pcal})
+    print(f"[bench] read until: plain form {plain['sim_seconds']} s", file=sys.stderr, flush=True)
     m.update({
         "config": "5 Mb reference (2 x 4,999,995 columns), 512 channels, 1600-sample chunks "
-                  "(0.4 s at 4 kHz), <= 4 chunks per read, FP16, median/MAD, 10% target reads, "
+                  f"(0.4 s at 4 kHz), <= 4 chunks per read, FP16, {norm} (first chunk), 10% target reads, "
                   "exponential read lengths (mean 20k samples) and gaps (mean 1 s)",
         "threshold_per_row": thr, "calibration_accuracy": cal_acc,
         "state_gb": round(info["state_bytes"] / 1e9, 2), "segment": info["segment"],
@@ -110,6 +120,8 @@ def read_until_leg(ctx, seconds):
                             "kernel_ms": round(kms, 2),
                             "kernel_gcups": round(cells / (kms * 1e-3) / 1e9, 1),
                             "chunks_per_s": round(512 / wall, 1)},
+        "plain_batched_form": plain,
+        "serving_vs_plain_reads_per_s": round(m["reads_decided_per_s"] / max(plain["reads_decided_per_s"], 1e-9), 3),
         "setup_s": round(time.time() - t0, 1),
     })
     return m

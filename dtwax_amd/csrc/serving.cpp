@@ -10,10 +10,16 @@
 // bit-identical to scoring the concatenated (first-chunk-normalised) query
 // from scratch with the same arithmetic.
 //
-// One batch = every chunk that is ready, one persistent sDTW launch over
+// A batch = chunks of distinct channels, one persistent sDTW launch over
 // (chunk x reference piece) work items.  The reference piece size is chosen
 // per batch from the batch size (state rows are indexed by absolute column,
 // so consecutive chunks of a read may use different pieces).
+//
+// Lanes: each lane is a HIP stream with its own staging buffers, work items,
+// keys and results.  The serving loop launches a new batch on a free lane
+// as soon as chunks arrive, while earlier batches are still running, so
+// their kernels overlap on the device (a read has at most one chunk in
+// flight; reads in flight hold distinct state slots).
 //
 // Device memory per slot: 2 rows (ping-pong: chunk k reads row cur and
 // writes row 1-cur, since the pieces of one chunk run concurrently and read
@@ -30,6 +36,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "dtwax.h"
@@ -54,8 +61,32 @@ struct ChanRec {
 struct Pending {
   int32_t channel;
   int32_t number;
+  bool release;  // end of the channel's read (no samples)
   std::vector<float> samples;
   Clock::time_point t;
+};
+
+struct Lane {
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+  float* d_raw = nullptr;
+  float* d_norm = nullptr;
+  int32_t* d_slot = nullptr;
+  int32_t* d_first = nullptr;
+  WorkItem* d_items = nullptr;
+  int32_t* d_counter = nullptr;
+  unsigned long long* d_keys = nullptr;
+  DeviceResult* d_res = nullptr;
+  void* d_scratch = nullptr;
+  float* h_raw = nullptr;
+  int32_t* h_slot = nullptr;
+  int32_t* h_first = nullptr;
+  WorkItem* h_items = nullptr;
+  DeviceResult* h_res = nullptr;
+  // batch in flight
+  bool busy = false;
+  std::vector<int32_t> channels;
+  std::vector<Clock::time_point> t_in;
 };
 
 constexpr uint32_t kHalf2Inf = 0x7c007c00u;
@@ -79,39 +110,23 @@ struct dtwax_ru {
   int64_t items_cap = 0;
   int64_t chunk_cap = 0;   // largest reference piece (columns)
   bool multiband = false;
-  // device
+  int64_t scratch_stride = 0;
+  // shared device state
   void* d_state = nullptr;
-  double* d_stats = nullptr;
-  float* d_raw = nullptr;
-  float* d_norm = nullptr;
+  double* d_stats = nullptr;  // (centre, inv) per slot
   int64_t* d_qoff = nullptr;
   int32_t* d_qlen = nullptr;
-  int32_t* d_slot = nullptr;
-  int32_t* d_first = nullptr;
-  WorkItem* d_items = nullptr;
-  int32_t* d_counter = nullptr;
-  unsigned long long* d_keys = nullptr;
-  DeviceResult* d_res = nullptr;
-  void* d_scratch = nullptr;
-  int64_t scratch_stride = 0;
-  hipStream_t st = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // pinned host staging
-  float* h_raw = nullptr;
-  int32_t* h_slot = nullptr;
-  int32_t* h_first = nullptr;
-  WorkItem* h_items = nullptr;
-  DeviceResult* h_res = nullptr;
-  // host state
+  std::vector<Lane> lanes;
+  // host bookkeeping (guarded by book_mu)
+  std::mutex book_mu;
   std::unordered_map<int32_t, ChanRec> chans;
   std::vector<int32_t> free_slots;
   std::vector<int32_t> slot_cur;
   int64_t batches = 0, chunks_scored = 0, cells = 0;
   double gpu_ms = 0.0;
-  std::mutex process_mu;
   // serving loop
   std::thread worker;
-  std::mutex mu;
+  std::mutex mu;  // inq, outq, flags
   std::condition_variable cv_in, cv_out;
   std::deque<Pending> inq;
   std::deque<dtwax_ru_decision> outq;
@@ -123,25 +138,57 @@ struct dtwax_ru {
 
 namespace {
 
+void lane_free(Lane& l) {
+  if (l.st) (void)hipStreamSynchronize(l.st);
+  for (void* p : {(void*)l.d_raw, (void*)l.d_norm, (void*)l.d_slot, (void*)l.d_first, (void*)l.d_items,
+                  (void*)l.d_counter, (void*)l.d_keys, (void*)l.d_res, l.d_scratch})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)l.h_raw, (void*)l.h_slot, (void*)l.h_first, (void*)l.h_items, (void*)l.h_res})
+    if (p) (void)hipHostFree(p);
+  for (hipEvent_t e : {l.ev0, l.ev1, l.done})
+    if (e) (void)hipEventDestroy(e);
+  if (l.st) (void)hipStreamDestroy(l.st);
+  l = Lane{};
+}
+
+bool lane_alloc(dtwax_ru* r, Lane& l) {
+  const size_t slots = (size_t)r->slots, L = (size_t)r->o.chunk_len;
+  bool ok = hipStreamCreate(&l.st) == hipSuccess && hipEventCreate(&l.ev0) == hipSuccess &&
+            hipEventCreate(&l.ev1) == hipSuccess &&
+            hipEventCreateWithFlags(&l.done, hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipMalloc(&l.d_raw, slots * L * sizeof(float)) == hipSuccess &&
+       hipMalloc(&l.d_norm, slots * L * sizeof(float)) == hipSuccess &&
+       hipMalloc(&l.d_slot, slots * sizeof(int32_t)) == hipSuccess &&
+       hipMalloc(&l.d_first, slots * sizeof(int32_t)) == hipSuccess &&
+       hipMalloc(&l.d_items, (size_t)r->items_cap * sizeof(WorkItem)) == hipSuccess &&
+       hipMalloc(&l.d_counter, sizeof(int32_t)) == hipSuccess &&
+       hipMalloc(&l.d_keys, slots * 2 * sizeof(unsigned long long)) == hipSuccess &&
+       hipMalloc(&l.d_res, slots * sizeof(DeviceResult)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&l.h_raw, slots * L * sizeof(float), 0) == hipSuccess &&
+       hipHostMalloc((void**)&l.h_slot, slots * sizeof(int32_t), 0) == hipSuccess &&
+       hipHostMalloc((void**)&l.h_first, slots * sizeof(int32_t), 0) == hipSuccess &&
+       hipHostMalloc((void**)&l.h_items, (size_t)r->items_cap * sizeof(WorkItem), 0) == hipSuccess &&
+       hipHostMalloc((void**)&l.h_res, slots * sizeof(DeviceResult), 0) == hipSuccess;
+  if (ok && r->multiband) {
+    const int64_t n = r->grid_slots * r->scratch_stride * r->epc;
+    ok = hipMalloc(&l.d_scratch, (size_t)n * 4) == hipSuccess &&
+         launch_fill_u32((uint32_t*)l.d_scratch, n, 0u, l.st) == hipSuccess;
+  }
+  return ok;
+}
+
 void ru_free(dtwax_ru* r) {
   if (r->ctx) (void)hipSetDevice(r->ctx->device);
-  if (r->st) (void)hipStreamSynchronize(r->st);
-  for (void* p : {r->d_state, (void*)r->d_stats, (void*)r->d_raw, (void*)r->d_norm, (void*)r->d_qoff,
-                  (void*)r->d_qlen, (void*)r->d_slot, (void*)r->d_first, (void*)r->d_items,
-                  (void*)r->d_counter, (void*)r->d_keys, (void*)r->d_res, r->d_scratch})
+  for (Lane& l : r->lanes) lane_free(l);
+  for (void* p : {r->d_state, (void*)r->d_stats, (void*)r->d_qoff, (void*)r->d_qlen})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)r->h_raw, (void*)r->h_slot, (void*)r->h_first, (void*)r->h_items, (void*)r->h_res})
-    if (p) (void)hipHostFree(p);
-  if (r->ev0) (void)hipEventDestroy(r->ev0);
-  if (r->ev1) (void)hipEventDestroy(r->ev1);
-  if (r->st) (void)hipStreamDestroy(r->st);
   delete r;
 }
 
 // Reference piece for a batch of m chunks: about 8 work items per resident
 // wave, never shorter than min_ref_chunk (the Q-1 overlap costs Q/piece).
 int64_t piece_for(const dtwax_ru* r, int64_t m) {
-  const int64_t N = r->ctx->n_cols;
+  const int64_t N = r->n_cols;
   const int64_t want_items = 8 * r->grid_slots;
   int64_t piece = (N * m + want_items - 1) / want_items;
   piece = std::max<int64_t>(piece, r->o.min_ref_chunk);
@@ -167,60 +214,59 @@ void release_slot(dtwax_ru* r, ChanRec& c) {
   c.slot = -1;
 }
 
-// One batch; the caller holds process_mu.
-int process_locked(dtwax_ru* r, const dtwax_ru_chunk* chunks, int32_t n, const float* samples,
-                   dtwax_ru_decision* out) {
+void release_channel(dtwax_ru* r, int32_t channel) {
+  auto it = r->chans.find(channel);
+  if (it != r->chans.end()) {
+    release_slot(r, it->second);
+    r->chans.erase(it);
+  }
+}
+
+// Admission of one chunk (book_mu held): true if it is to be scored (the
+// read holds a slot), false if it is answered at once (NO_SLOT or STALE).
+bool admit(dtwax_ru* r, int32_t channel, int32_t number, dtwax_ru_decision& d) {
+  auto it = r->chans.find(channel);
+  if (it == r->chans.end() || it->second.number != number) {
+    if (it != r->chans.end()) release_slot(r, it->second);
+    ChanRec fresh;
+    fresh.number = number;
+    if (r->free_slots.empty()) {
+      fresh.action = DTWAX_RU_NO_SLOT;
+      r->chans[channel] = fresh;
+      set_decision(d, channel, fresh, DTWAX_RU_NO_SLOT);
+      return false;
+    }
+    fresh.slot = r->free_slots.back();
+    r->free_slots.pop_back();
+    r->chans[channel] = fresh;
+    return true;
+  }
+  if (it->second.action != DTWAX_RU_CONTINUE) {
+    set_decision(d, channel, it->second, DTWAX_RU_STALE);
+    return false;
+  }
+  return true;
+}
+
+// Stages admitted chunks (channels[j], samples src[j]) and launches the
+// batch asynchronously on lane l (book_mu held).
+int launch(dtwax_ru* r, Lane& l, const std::vector<int32_t>& channels, const std::vector<const float*>& src) {
   dtwax_ctx* c = r->ctx;
-  const int32_t L = r->o.chunk_len;
   if (c->n_cols != r->n_cols || c->n_strands != r->n_strands)
     return set_error(DTWAX_ENOTREADY, "the context's reference changed after dtwax_ru_create");
-  // pass 1: admission and channel bookkeeping
-  std::vector<int32_t> scored;  // indices into chunks
-  scored.reserve((size_t)n);
-  {
-    std::unordered_map<int32_t, int32_t> seen;
-    for (int32_t i = 0; i < n; i++)
-      if (!seen.emplace(chunks[i].channel, i).second)
-        return set_error(DTWAX_EINVAL, "two chunks of one channel in one batch");
-  }
-  for (int32_t i = 0; i < n; i++) {
-    const dtwax_ru_chunk& ch = chunks[i];
-    auto it = r->chans.find(ch.channel);
-    if (it == r->chans.end() || it->second.number != ch.number) {
-      if (it != r->chans.end()) release_slot(r, it->second);
-      ChanRec fresh;
-      fresh.number = ch.number;
-      if (r->free_slots.empty()) {
-        fresh.action = DTWAX_RU_NO_SLOT;
-        r->chans[ch.channel] = fresh;
-        set_decision(out[i], ch.channel, fresh, DTWAX_RU_NO_SLOT);
-        continue;
-      }
-      fresh.slot = r->free_slots.back();
-      r->free_slots.pop_back();
-      r->chans[ch.channel] = fresh;
-    } else if (it->second.action != DTWAX_RU_CONTINUE) {
-      set_decision(out[i], ch.channel, it->second, DTWAX_RU_STALE);
-      continue;
-    }
-    scored.push_back(i);
-  }
-  const int64_t m = (int64_t)scored.size();
-  if (m == 0) return DTWAX_OK;
-
-  // pass 2: stage samples and work items
-  const int64_t N = c->n_cols;
+  const int32_t L = r->o.chunk_len;
+  const int64_t m = (int64_t)channels.size();
+  const int64_t N = r->n_cols;
   const int64_t piece = piece_for(r, m);
   const int64_t per_read = (N + piece - 1) / piece;
-  if (m * per_read > r->items_cap) return set_error(DTWAX_EINVAL, "work item capacity exceeded");
+  if (m > r->slots || m * per_read > r->items_cap) return set_error(DTWAX_EINVAL, "batch capacity exceeded");
   int64_t ni = 0;
   for (int64_t j = 0; j < m; j++) {
-    const dtwax_ru_chunk& ch = chunks[scored[(size_t)j]];
-    const ChanRec& rec = r->chans[ch.channel];
-    std::memcpy(r->h_raw + j * L, samples + ch.offset, (size_t)L * sizeof(float));
-    r->h_slot[j] = rec.slot;
+    const ChanRec& rec = r->chans[channels[(size_t)j]];
+    std::memcpy(l.h_raw + j * L, src[(size_t)j], (size_t)L * sizeof(float));
+    l.h_slot[j] = rec.slot;
     const bool first = rec.chunks == 0;
-    r->h_first[j] = first ? 1 : 0;
+    l.h_first[j] = first ? 1 : 0;
     const bool store = rec.chunks + 1 < r->o.max_chunks;
     for (int64_t c0 = 0; c0 < N; c0 += piece) {
       WorkItem w{};
@@ -231,51 +277,61 @@ int process_locked(dtwax_ru* r, const dtwax_ru_chunk* chunks, int32_t n, const f
       w.x = (int32_t)std::max<int64_t>(0, c0 - (L - 1));
       w.flags = (first ? 0 : ITEM_RESUME) | (store ? ITEM_STORE : 0);
       w.cur = r->slot_cur[(size_t)rec.slot];
-      r->h_items[ni++] = w;
+      l.h_items[ni++] = w;
     }
   }
   DTWAX_HIP_TRY(hipSetDevice(c->device));
-  DTWAX_HIP_TRY(hipMemcpyAsync(r->d_raw, r->h_raw, (size_t)(m * L) * sizeof(float), hipMemcpyHostToDevice, r->st));
-  DTWAX_HIP_TRY(hipMemcpyAsync(r->d_slot, r->h_slot, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, r->st));
-  DTWAX_HIP_TRY(hipMemcpyAsync(r->d_first, r->h_first, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, r->st));
-  DTWAX_HIP_TRY(hipMemcpyAsync(r->d_items, r->h_items, (size_t)ni * sizeof(WorkItem), hipMemcpyHostToDevice, r->st));
-  DTWAX_HIP_TRY(launch_ru_normalize(r->d_raw, (int32_t)m, L, r->d_slot, r->d_first, r->o.norm, r->o.eps,
-                                    r->d_stats, r->d_norm, r->st));
+  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_raw, l.h_raw, (size_t)(m * L) * sizeof(float), hipMemcpyHostToDevice, l.st));
+  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_slot, l.h_slot, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, l.st));
+  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_first, l.h_first, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, l.st));
+  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_items, l.h_items, (size_t)ni * sizeof(WorkItem), hipMemcpyHostToDevice, l.st));
+  DTWAX_HIP_TRY(launch_ru_normalize(l.d_raw, (int32_t)m, L, l.d_slot, l.d_first, r->o.norm, r->o.eps,
+                                    r->d_stats, l.d_norm, l.st));
   ScoreLaunch sl;
   sl.precision = r->o.precision;
   sl.S = r->S;
   sl.nstr = r->nstr;
   sl.grid = (int)std::min<int64_t>(r->grid_slots, ni);
-  sl.queries = r->d_norm;
+  sl.queries = l.d_norm;
   sl.q_off = r->d_qoff;
   sl.q_len = r->d_qlen;
-  sl.items = r->d_items;
+  sl.items = l.d_items;
   sl.n_items = (int32_t)ni;
   sl.n_reads = (int32_t)m;
-  sl.counter = r->d_counter;
-  sl.keys = r->d_keys;
-  sl.scratch = r->d_scratch;
+  sl.counter = l.d_counter;
+  sl.keys = l.d_keys;
+  sl.scratch = l.d_scratch;
   sl.scratch_stride = r->scratch_stride;
   sl.state = (char*)r->d_state + (int64_t)COL_PADL * r->epc * 4;  // column 0 of slot 0, row 0
   sl.row_stride = r->row_stride;
-  sl.out = r->d_res;
-  sl.ev0 = r->ev0;
-  sl.ev1 = r->ev1;
-  int rc = run_score_launch(c, sl, r->st);
+  sl.out = l.d_res;
+  sl.ev0 = l.ev0;
+  sl.ev1 = l.ev1;
+  int rc = run_score_launch(c, sl, l.st);
   if (rc) return rc;
-  DTWAX_HIP_TRY(hipMemcpyAsync(r->h_res, r->d_res, (size_t)m * sizeof(DeviceResult), hipMemcpyDeviceToHost, r->st));
-  DTWAX_HIP_TRY(hipStreamSynchronize(r->st));
+  DTWAX_HIP_TRY(hipMemcpyAsync(l.h_res, l.d_res, (size_t)m * sizeof(DeviceResult), hipMemcpyDeviceToHost, l.st));
+  DTWAX_HIP_TRY(hipEventRecord(l.done, l.st));
+  l.channels = channels;
+  l.busy = true;
+  return DTWAX_OK;
+}
+
+// Waits for lane l's batch and makes the decisions, out[j] for channels[j]
+// (book_mu held).
+int retire(dtwax_ru* r, Lane& l, dtwax_ru_decision* out) {
+  const int32_t L = r->o.chunk_len;
+  DTWAX_HIP_TRY(hipEventSynchronize(l.done));
+  l.busy = false;
+  const int64_t m = (int64_t)l.channels.size();
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, r->ev0, r->ev1) == hipSuccess) r->gpu_ms += ms;
+  if (hipEventElapsedTime(&ms, l.ev0, l.ev1) == hipSuccess) r->gpu_ms += ms;
   r->batches++;
   r->chunks_scored += m;
-  r->cells += m * (int64_t)L * N * r->nstr;
-
-  // pass 3: decisions
+  r->cells += m * (int64_t)L * r->n_cols * r->nstr;
   for (int64_t j = 0; j < m; j++) {
-    const int32_t i = scored[(size_t)j];
-    ChanRec& rec = r->chans[chunks[i].channel];
-    const DeviceResult& res = r->h_res[j];
+    const int32_t ch = l.channels[(size_t)j];
+    ChanRec& rec = r->chans[ch];
+    const DeviceResult& res = l.h_res[j];
     rec.chunks++;
     for (int k = 0; k < 2; k++) {
       rec.score[k] = (k < r->nstr) ? res.score[k] : INFINITY;
@@ -294,65 +350,116 @@ int process_locked(dtwax_ru* r, const dtwax_ru_chunk* chunks, int32_t n, const f
       rec.action = action;
       release_slot(r, rec);
     }
-    set_decision(out[i], chunks[i].channel, rec, action);
+    set_decision(out[j], ch, rec, action);
   }
   return DTWAX_OK;
 }
 
+void worker_fail(dtwax_ru* r, int rc) {
+  std::lock_guard<std::mutex> lk(r->mu);
+  r->worker_rc = rc;
+  r->worker_err = dtwax_last_error();
+  r->running = false;
+  r->cv_out.notify_all();
+}
+
 void worker_main(dtwax_ru* r) {
-  const int32_t L = r->o.chunk_len;
-  std::vector<dtwax_ru_chunk> batch;
-  std::vector<float> samples;
-  std::vector<Clock::time_point> t_in;
+  std::unordered_set<int32_t> inflight;  // channels with a chunk on the GPU
   std::vector<dtwax_ru_decision> dec;
   for (;;) {
+    bool progressed = false;
+    // 1. retire finished batches
+    for (Lane& l : r->lanes) {
+      if (!l.busy) continue;
+      const hipError_t q = hipEventQuery(l.done);
+      if (q == hipErrorNotReady) continue;
+      dec.assign(l.channels.size(), dtwax_ru_decision{});
+      int rc;
+      {
+        std::lock_guard<std::mutex> bk(r->book_mu);
+        rc = (q == hipSuccess) ? retire(r, l, dec.data())
+                               : set_error(DTWAX_EDEVICE, std::string("batch failed: ") + hipGetErrorString(q));
+      }
+      if (rc) return worker_fail(r, rc);
+      const Clock::time_point now = Clock::now();
+      std::lock_guard<std::mutex> lk(r->mu);
+      for (size_t j = 0; j < dec.size(); j++) {
+        dec[j].latency_us = std::chrono::duration<double, std::micro>(now - l.t_in[j]).count();
+        inflight.erase(l.channels[j]);
+        r->outq.push_back(dec[j]);
+      }
+      r->cv_out.notify_all();
+      progressed = true;
+    }
+    // 2. launch what has arrived on a free lane: the earliest chunk of each
+    //    channel without a chunk in flight; later ones keep their order
+    Lane* free_lane = nullptr;
+    for (Lane& l : r->lanes)
+      if (!l.busy) { free_lane = &l; break; }
     std::vector<Pending> take;
+    bool idle_exit = false;
     {
       std::unique_lock<std::mutex> lk(r->mu);
-      r->cv_in.wait(lk, [&] { return r->stop_req || !r->inq.empty(); });
-      if (r->inq.empty() && r->stop_req) return;
-      // every ready chunk, at most one per channel (later ones keep their order)
-      std::unordered_map<int32_t, int> seen;
-      std::deque<Pending> keep;
-      while (!r->inq.empty()) {
-        Pending p = std::move(r->inq.front());
-        r->inq.pop_front();
-        if (seen.emplace(p.channel, 1).second) take.push_back(std::move(p));
-        else keep.push_back(std::move(p));
+      bool any_busy = false;
+      for (Lane& l : r->lanes) any_busy |= l.busy;
+      if (free_lane && !r->inq.empty()) {
+        std::unordered_set<int32_t> seen;
+        std::deque<Pending> keep;
+        while (!r->inq.empty()) {
+          Pending p = std::move(r->inq.front());
+          r->inq.pop_front();
+          if (inflight.count(p.channel) || !seen.insert(p.channel).second) keep.push_back(std::move(p));
+          else take.push_back(std::move(p));
+        }
+        r->inq.swap(keep);
       }
-      r->inq.swap(keep);
+      if (take.empty() && !progressed) {
+        if (r->stop_req && r->inq.empty() && !any_busy) idle_exit = true;
+        else if (!any_busy && r->inq.empty()) r->cv_in.wait(lk, [&] { return r->stop_req || !r->inq.empty(); });
+        else r->cv_in.wait_for(lk, std::chrono::microseconds(50));
+      }
     }
-    const int32_t n = (int32_t)take.size();
-    batch.resize((size_t)n);
-    samples.resize((size_t)n * L);
-    dec.assign((size_t)n, dtwax_ru_decision{});
-    for (int32_t i = 0; i < n; i++) {
-      batch[(size_t)i].channel = take[(size_t)i].channel;
-      batch[(size_t)i].number = take[(size_t)i].number;
-      batch[(size_t)i].offset = (int64_t)i * L;
-      std::memcpy(samples.data() + (size_t)i * L, take[(size_t)i].samples.data(), (size_t)L * sizeof(float));
-    }
-    int rc;
+    if (idle_exit) return;
+    if (take.empty()) continue;
+    // admission; releases and answered chunks go straight out
+    std::vector<int32_t> channels;
+    std::vector<const float*> src;
+    std::vector<Clock::time_point> t_in;
+    std::vector<dtwax_ru_decision> answered;
+    int rc = DTWAX_OK;
     {
-      std::lock_guard<std::mutex> lk(r->process_mu);
-      rc = process_locked(r, batch.data(), n, samples.data(), dec.data());
+      std::lock_guard<std::mutex> bk(r->book_mu);
+      for (Pending& p : take) {
+        if (p.release) {
+          release_channel(r, p.channel);
+          continue;
+        }
+        dtwax_ru_decision d{};
+        if (admit(r, p.channel, p.number, d)) {
+          channels.push_back(p.channel);
+          src.push_back(p.samples.data());
+          t_in.push_back(p.t);
+        } else {
+          d.latency_us = std::chrono::duration<double, std::micro>(Clock::now() - p.t).count();
+          answered.push_back(d);
+        }
+      }
+      if (!channels.empty()) rc = launch(r, *free_lane, channels, src);
     }
-    const Clock::time_point now = Clock::now();
+    if (rc) return worker_fail(r, rc);
     std::lock_guard<std::mutex> lk(r->mu);
-    if (rc != DTWAX_OK) {
-      r->worker_rc = rc;
-      r->worker_err = dtwax_last_error();
-      r->running = false;
-      r->cv_out.notify_all();
-      return;
+    if (!channels.empty()) {
+      free_lane->t_in = std::move(t_in);
+      for (int32_t ch : channels) inflight.insert(ch);
     }
-    for (int32_t i = 0; i < n; i++) {
-      dec[(size_t)i].latency_us =
-          std::chrono::duration<double, std::micro>(now - take[(size_t)i].t).count();
-      r->outq.push_back(dec[(size_t)i]);
-    }
-    r->cv_out.notify_all();
+    for (auto& d : answered) r->outq.push_back(d);
+    if (!answered.empty()) r->cv_out.notify_all();
   }
+}
+
+bool loop_running(dtwax_ru* r) {
+  std::lock_guard<std::mutex> lk(r->mu);
+  return r->running;
 }
 
 }  // namespace
@@ -367,8 +474,10 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   if (o.max_reads == 0) o.max_reads = 512;
   if (o.chunk_len == 0) o.chunk_len = 1600;
   if (o.max_chunks == 0) o.max_chunks = 4;
-  if (o.max_reads < 1 || o.chunk_len < 64 || o.max_chunks < 1)
-    return set_error(DTWAX_EINVAL, "max_reads, chunk_len (>= 64) and max_chunks must be positive");
+  if (o.n_streams == 0) o.n_streams = 2;
+  if (o.max_reads < 1 || o.chunk_len < 64 || o.max_chunks < 1 || o.n_streams < 1 || o.n_streams > 8)
+    return set_error(DTWAX_EINVAL, "max_reads, chunk_len (>= 64), max_chunks and n_streams (1..8) "
+                                   "must be positive");
   if (o.precision != DTWAX_F16 && o.precision != DTWAX_F32)
     return set_error(DTWAX_EINVAL, "Read Until serving supports DTWAX_F16 and DTWAX_F32");
   if (o.norm < DTWAX_NORM_NONE || o.norm > DTWAX_NORM_MEDMAD) return set_error(DTWAX_EINVAL, "bad norm");
@@ -411,53 +520,32 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   if (hipSetDevice(c->device) != hipSuccess) return cleanup(DTWAX_EDEVICE, "hipSetDevice");
   r->grid_slots = (int64_t)std::max(1, occupancy(o.precision, r->S, r->nstr, false)) * c->n_cu;
   const int64_t N = c->n_cols;
+  const int64_t L = o.chunk_len;
   // band scratch only for chunks taller than one band: cap the piece there
   r->chunk_cap = r->multiband ? std::min<int64_t>(N, std::max<int64_t>(o.min_ref_chunk, 65536)) : N;
   const int64_t min_piece = std::min<int64_t>(N, std::max<int64_t>(64, (int64_t)o.min_ref_chunk / 64 * 64));
   r->items_cap = slots * ((N + min_piece - 1) / min_piece);
-  const int64_t L = o.chunk_len;
-  bool ok = hipStreamCreate(&r->st) == hipSuccess && hipEventCreate(&r->ev0) == hipSuccess &&
-            hipEventCreate(&r->ev1) == hipSuccess;
-  if (!ok) return cleanup(DTWAX_EDEVICE, "stream/event creation failed");
-  ok = hipMalloc(&r->d_state, (size_t)(slots * slot_bytes)) == hipSuccess &&
-       hipMalloc(&r->d_stats, (size_t)slots * 2 * sizeof(double)) == hipSuccess &&
-       hipMalloc(&r->d_raw, (size_t)(slots * L) * sizeof(float)) == hipSuccess &&
-       hipMalloc(&r->d_norm, (size_t)(slots * L) * sizeof(float)) == hipSuccess &&
-       hipMalloc(&r->d_qoff, (size_t)slots * sizeof(int64_t)) == hipSuccess &&
-       hipMalloc(&r->d_qlen, (size_t)slots * sizeof(int32_t)) == hipSuccess &&
-       hipMalloc(&r->d_slot, (size_t)slots * sizeof(int32_t)) == hipSuccess &&
-       hipMalloc(&r->d_first, (size_t)slots * sizeof(int32_t)) == hipSuccess &&
-       hipMalloc(&r->d_items, (size_t)r->items_cap * sizeof(WorkItem)) == hipSuccess &&
-       hipMalloc(&r->d_counter, sizeof(int32_t)) == hipSuccess &&
-       hipMalloc(&r->d_keys, (size_t)slots * 2 * sizeof(unsigned long long)) == hipSuccess &&
-       hipMalloc(&r->d_res, (size_t)slots * sizeof(DeviceResult)) == hipSuccess;
+  if (r->multiband) r->scratch_stride = COL_PADL + r->chunk_cap + L + COL_PADR;
+  bool ok = hipMalloc(&r->d_state, (size_t)(slots * slot_bytes)) == hipSuccess &&
+            hipMalloc(&r->d_stats, (size_t)slots * 2 * sizeof(double)) == hipSuccess &&
+            hipMalloc(&r->d_qoff, (size_t)slots * sizeof(int64_t)) == hipSuccess &&
+            hipMalloc(&r->d_qlen, (size_t)slots * sizeof(int32_t)) == hipSuccess;
   if (!ok) return cleanup(DTWAX_ENOMEM, "device allocation failed (state " + std::to_string(slots * slot_bytes) + " bytes)");
-  ok = hipHostMalloc((void**)&r->h_raw, (size_t)(slots * L) * sizeof(float), 0) == hipSuccess &&
-       hipHostMalloc((void**)&r->h_slot, (size_t)slots * sizeof(int32_t), 0) == hipSuccess &&
-       hipHostMalloc((void**)&r->h_first, (size_t)slots * sizeof(int32_t), 0) == hipSuccess &&
-       hipHostMalloc((void**)&r->h_items, (size_t)r->items_cap * sizeof(WorkItem), 0) == hipSuccess &&
-       hipHostMalloc((void**)&r->h_res, (size_t)slots * sizeof(DeviceResult), 0) == hipSuccess;
-  if (!ok) return cleanup(DTWAX_ENOMEM, "pinned host allocation failed");
-  if (r->multiband) {
-    r->scratch_stride = COL_PADL + r->chunk_cap + L + COL_PADR;
-    const size_t bytes = (size_t)r->grid_slots * (size_t)r->scratch_stride * (size_t)r->epc * 4;
-    if (hipMalloc(&r->d_scratch, bytes) != hipSuccess) return cleanup(DTWAX_ENOMEM, "band scratch allocation failed");
-  }
+  r->lanes.resize((size_t)o.n_streams);
+  for (Lane& l : r->lanes)
+    if (!lane_alloc(r, l)) return cleanup(DTWAX_ENOMEM, "lane allocation failed");
   // state rows start at +inf everywhere: the left pad is column -1 of every
   // stored row (+inf by the recurrence), the rest is overwritten before use
   if (launch_fill_u32((uint32_t*)r->d_state, slots * slot_bytes / 4,
-                      o.precision == DTWAX_F32 ? kFloatInf : kHalf2Inf, r->st) != hipSuccess)
+                      o.precision == DTWAX_F32 ? kFloatInf : kHalf2Inf, r->lanes[0].st) != hipSuccess)
     return cleanup(DTWAX_EDEVICE, "state fill failed");
-  if (r->d_scratch &&
-      launch_fill_u32((uint32_t*)r->d_scratch, r->grid_slots * r->scratch_stride * r->epc, 0u, r->st) != hipSuccess)
-    return cleanup(DTWAX_EDEVICE, "scratch fill failed");
   {
     std::vector<int64_t> off((size_t)slots);
     std::vector<int32_t> len((size_t)slots, (int32_t)L);
     for (int64_t j = 0; j < slots; j++) off[(size_t)j] = j * L;
     if (hipMemcpy(r->d_qoff, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(r->d_qlen, len.data(), len.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipStreamSynchronize(r->st) != hipSuccess)
+        hipDeviceSynchronize() != hipSuccess)
       return cleanup(DTWAX_EDEVICE, "initial upload failed");
   }
   r->free_slots.reserve((size_t)slots);
@@ -474,8 +562,9 @@ int dtwax_ru_destroy(dtwax_ru* r) {
   return DTWAX_OK;
 }
 
-int dtwax_ru_info(const dtwax_ru* r, int32_t* slots, int32_t* active, int32_t* segment, int64_t* state_bytes) {
+int dtwax_ru_info(dtwax_ru* r, int32_t* slots, int32_t* active, int32_t* segment, int64_t* state_bytes) {
   if (!r) return set_error(DTWAX_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> bk(r->book_mu);
   if (slots) *slots = r->slots;
   if (active) *active = r->slots - (int32_t)r->free_slots.size();
   if (segment) *segment = r->S;
@@ -483,8 +572,9 @@ int dtwax_ru_info(const dtwax_ru* r, int32_t* slots, int32_t* active, int32_t* s
   return DTWAX_OK;
 }
 
-int dtwax_ru_stats(const dtwax_ru* r, int64_t* batches, int64_t* chunks, int64_t* cells, double* gpu_ms) {
+int dtwax_ru_stats(dtwax_ru* r, int64_t* batches, int64_t* chunks, int64_t* cells, double* gpu_ms) {
   if (!r) return set_error(DTWAX_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> bk(r->book_mu);
   if (batches) *batches = r->batches;
   if (chunks) *chunks = r->chunks_scored;
   if (cells) *cells = r->cells;
@@ -495,21 +585,54 @@ int dtwax_ru_stats(const dtwax_ru* r, int64_t* batches, int64_t* chunks, int64_t
 int dtwax_ru_process(dtwax_ru* r, const dtwax_ru_chunk* chunks, int32_t n, const float* samples,
                      dtwax_ru_decision* out) {
   if (!r || n < 0 || (n && (!chunks || !samples || !out))) return set_error(DTWAX_EINVAL, "NULL argument");
-  for (int32_t i = 0; i < n; i++)
-    if (chunks[i].offset < 0 || chunks[i].channel < 0)
-      return set_error(DTWAX_EINVAL, "negative channel or offset");
-  std::lock_guard<std::mutex> lk(r->process_mu);
-  return process_locked(r, chunks, n, samples, out);
+  if (loop_running(r)) return set_error(DTWAX_EINVAL, "the serving loop is running: use dtwax_ru_submit");
+  {
+    std::unordered_set<int32_t> seen;
+    for (int32_t i = 0; i < n; i++) {
+      if (chunks[i].offset < 0 || chunks[i].channel < 0)
+        return set_error(DTWAX_EINVAL, "negative channel or offset");
+      if (!seen.insert(chunks[i].channel).second)
+        return set_error(DTWAX_EINVAL, "two chunks of one channel in one batch");
+    }
+  }
+  std::lock_guard<std::mutex> bk(r->book_mu);
+  std::vector<int32_t> idx, channels;
+  std::vector<const float*> src;
+  for (int32_t i = 0; i < n; i++) {
+    if (admit(r, chunks[i].channel, chunks[i].number, out[i])) {
+      idx.push_back(i);
+      channels.push_back(chunks[i].channel);
+      src.push_back(samples + chunks[i].offset);
+    }
+  }
+  if (channels.empty()) return DTWAX_OK;
+  Lane& l = r->lanes[0];
+  int rc = launch(r, l, channels, src);
+  if (rc) return rc;
+  std::vector<dtwax_ru_decision> dec(channels.size());
+  rc = retire(r, l, dec.data());
+  if (rc) return rc;
+  for (size_t j = 0; j < idx.size(); j++) out[idx[j]] = dec[j];
+  return DTWAX_OK;
 }
 
 int dtwax_ru_release(dtwax_ru* r, int32_t channel) {
   if (!r) return set_error(DTWAX_EINVAL, "NULL argument");
-  std::lock_guard<std::mutex> lk(r->process_mu);
-  auto it = r->chans.find(channel);
-  if (it != r->chans.end()) {
-    release_slot(r, it->second);
-    r->chans.erase(it);
+  {
+    std::lock_guard<std::mutex> lk(r->mu);
+    if (r->running) {  // in order with the channel's queued chunks
+      Pending p;
+      p.channel = channel;
+      p.number = 0;
+      p.release = true;
+      p.t = Clock::now();
+      r->inq.push_back(std::move(p));
+      r->cv_in.notify_one();
+      return DTWAX_OK;
+    }
   }
+  std::lock_guard<std::mutex> bk(r->book_mu);
+  release_channel(r, channel);
   return DTWAX_OK;
 }
 
@@ -530,12 +653,13 @@ int dtwax_ru_submit(dtwax_ru* r, int32_t channel, int32_t number, const float* s
   Pending p;
   p.channel = channel;
   p.number = number;
+  p.release = false;
   p.samples.assign(samples, samples + r->o.chunk_len);
   p.t = Clock::now();
   std::lock_guard<std::mutex> lk(r->mu);
-  if (!r->running) return set_error(r->worker_rc ? r->worker_rc : DTWAX_ENOTREADY,
-                                    r->worker_rc ? "serving loop failed: " + r->worker_err
-                                                 : "serving loop not started");
+  if (!r->running)
+    return set_error(r->worker_rc ? r->worker_rc : DTWAX_ENOTREADY,
+                     r->worker_rc ? "serving loop failed: " + r->worker_err : "serving loop not started");
   r->inq.push_back(std::move(p));
   r->cv_in.notify_one();
   return DTWAX_OK;

@@ -1,5 +1,5 @@
 """Read Until serving simulation: a flow cell of channels streaming chunks
-into the dtwax_ru_* engine, replayed event by event.
+into the dtwax_ru_* engine.
 
 The reference's own serving path is a prototype: ont_readuntil.py feeds the
 last 16,000 bytes of every 2nd read into a FIFO and simulate_minion.py
@@ -11,15 +11,25 @@ the stateful engine instead:
   exponential gap (mean gap_s) and have exponential lengths (mean
   read_mean samples, at least one chunk); target_frac of them come from the
   reference (random strand and position), the rest from unrelated sequence;
-* every chunk_len samples a read delivers one chunk, until the engine
-  decides it; a NONTARGET read is unblocked (ends unblock_s after the
-  decision), a TARGET read sequences to its natural end;
-* the server takes every chunk that has arrived whenever it is idle (one
-  chunk per channel per batch) and calls ReadUntil.process; simulated time
-  advances by the measured wall time of that call, so latencies are real
-  GPU + host processing times with simulated arrivals.
+* every chunk_len samples a read delivers one chunk, until it is decided;
+  a NONTARGET read is unblocked (ends unblock_s after the decision), a
+  TARGET read sequences to its natural end.
 
-Latency = decision available - chunk arrival.
+Three drivers:
+* ``simulate`` -- the serving form, replayed event by event: whenever the
+  server is idle it scores every chunk that has arrived (one per channel)
+  with ReadUntil.process; simulated time advances by the measured wall time
+  of the call (deterministic arrivals, real processing times);
+* ``simulate_realtime`` -- the serving form in wall-clock time through the
+  engine's own serving loop (submit/poll, overlapping batches on several
+  streams); latencies are measured inside the engine;
+* ``simulate_plain`` -- the plain batched form (the reference's way): a read
+  is classified once, from scratch, after all max_chunks chunks have
+  arrived, in batches of whatever reads are complete when the server is
+  idle, with the batch scoring path.
+
+Latency = decision available - chunk arrival (for the plain form: the
+arrival of the read's last chunk).
 """
 from __future__ import annotations
 
@@ -30,6 +40,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import synth
+
+CONTINUE, NONTARGET, TARGET = 0, 1, 2
 
 
 @dataclass
@@ -67,151 +79,293 @@ class ReadSource:
         return synth.simulate_signal(self.rng, lv, n_samples)
 
 
-def calibrate_threshold_per_row(ctx, source, chunk_len, max_chunks, precision="f16", n_each=32):
-    """Per-row threshold from n_each target and n_each non-target reads
-    scored chunk by chunk by an engine with the threshold off: the midpoint
-    between the largest target and the smallest non-target score per row
-    over every chunk count, or the accuracy-maximising cut if they overlap.
+def _best_cut(per_row, truth):
+    """Midpoint between the classes if they separate, else the cut with the
+    best balanced accuracy (target iff value < cut)."""
+    t_hi = per_row[..., truth].max()
+    n_lo = per_row[..., ~truth].min()
+    if t_hi < n_lo:
+        return float(0.5 * (t_hi + n_lo)), 1.0
+    best, acc = float(n_lo), 0.0
+    for c in np.unique(per_row):
+        a = ((per_row[..., truth] < c).mean() + (per_row[..., ~truth] >= c).mean()) / 2
+        if a > acc:
+            best, acc = float(c), float(a)
+    return best, acc
+
+
+def calibrate_threshold_per_row(ctx, source, chunk_len, max_chunks, precision="f16", n_each=32,
+                                norm=None):
+    """Serving form: per-row threshold from n_each target and n_each
+    non-target reads scored chunk by chunk by an engine with the threshold
+    off.  Under the engine's rule a read is TARGET iff its per-row score
+    stays below the threshold at every chunk, i.e. iff the largest per-row
+    score over its chunks is below it, so the cut is made on that maximum.
     Returns (threshold_per_row, calibration accuracy)."""
-    from . import ReadUntil
+    from . import NORM_MEDMAD, ReadUntil
     n = 2 * n_each
     truth = np.array([True] * n_each + [False] * n_each)
     sig = np.stack([source.signal(chunk_len * max_chunks, t) for t in truth])
     per_row = np.zeros((max_chunks, n))
     ru = ReadUntil(ctx, max_reads=n, chunk_len=chunk_len, max_chunks=max_chunks,
-                   precision=precision)
+                   precision=precision, norm=NORM_MEDMAD if norm is None else norm)
     try:
         for k in range(max_chunks):
             dec = ru.process(np.arange(n), np.full(n, 1), sig[:, k * chunk_len:(k + 1) * chunk_len])
             per_row[k] = dec["score"].min(axis=1) / ((k + 1) * chunk_len)
     finally:
         ru.close()
-    t_hi = per_row[:, truth].max()
-    n_lo = per_row[:, ~truth].min()
-    if t_hi < n_lo:
-        return float(0.5 * (t_hi + n_lo)), 1.0
-    cand = np.unique(per_row)
-    best, acc = float(cand[0]), 0.0
-    for c in cand:
-        a = ((per_row[:, truth] < c).mean() + (per_row[:, ~truth] >= c).mean()) / 2
-        if a > acc:
-            best, acc = float(c), a
-    return best, float(acc)
+    return _best_cut(per_row.max(axis=0), truth)
+
+
+def _plain_scores(ctx, raws, precision, norm=None):
+    """Plain batched form: statistics of each whole read, then the batch
+    scoring path from scratch; min(fwd, rev) per read."""
+    from . import NORM_MEDMAD
+    normed, _ = ctx.normalize(raws, mode=NORM_MEDMAD if norm is None else norm)
+    sc, _ = ctx.score(normed, precision=precision)
+    return sc.min(axis=1)
+
+
+def calibrate_plain_threshold(ctx, source, n_samples, precision="f16", n_each=32, norm=None):
+    """Plain form: per-row threshold on whole n_samples reads."""
+    truth = np.array([True] * n_each + [False] * n_each)
+    raws = [source.signal(n_samples, t) for t in truth]
+    per_row = _plain_scores(ctx, raws, precision, norm) / n_samples
+    return _best_cut(per_row, truth)
+
+
+class FlowCell:
+    """Channels, reads and their event queue (start, chunk, end)."""
+
+    def __init__(self, source, cfg: SimConfig):
+        self.src, self.cfg = source, cfg
+        self.rng = np.random.default_rng(cfg.seed)
+        self.L, self.fs = cfg.chunk_len, cfg.sample_rate
+        self.chunk_s = self.L / self.fs
+        self.events, self.seq = [], 0
+        self.reads = {}  # (channel, number) -> dict
+        self.number = np.zeros(cfg.n_channels, np.int64)
+        self.decided = []
+        for ch in range(cfg.n_channels):
+            self.push(float(self.rng.exponential(cfg.gap_s)), "start", ch)
+
+    def push(self, t, kind, ch, payload=None):
+        heapq.heappush(self.events, (t, self.seq, kind, ch, payload))
+        self.seq += 1
+
+    def start_read(self, ch, t0):
+        cfg, L = self.cfg, self.L
+        self.number[ch] += 1
+        num = int(self.number[ch])
+        n = max(L, int(self.rng.exponential(cfg.read_mean)))
+        target = bool(self.rng.random() < cfg.target_frac)
+        n_chunks = min(cfg.max_chunks, n // L)
+        r = dict(t0=t0, n=n, target=target, end=t0 + n / self.fs, decided=None,
+                 sig=self.src.signal(n_chunks * L, target), n_chunks=n_chunks, ended=False)
+        self.reads[(ch, num)] = r
+        for k in range(n_chunks):
+            self.push(t0 + (k + 1) * self.chunk_s, "chunk", ch, (num, k))
+        self.push(r["end"], "end", ch, num)
+
+    def next_time(self):
+        return self.events[0][0] if self.events else None
+
+    def pop_due(self, now):
+        """Yields due events after applying start/end bookkeeping:
+        ("chunk", t, ch, num, k) for live reads, ("end", t, ch, num)."""
+        while self.events and self.events[0][0] <= now:
+            t, _, kind, ch, payload = heapq.heappop(self.events)
+            if kind == "start":
+                if t < self.cfg.duration_s:
+                    self.start_read(ch, t)
+            elif kind == "chunk":
+                num, k = payload
+                r = self.reads[(ch, num)]
+                if r["decided"] is None and not r["ended"]:
+                    yield ("chunk", t, ch, num, k)
+            else:
+                r = self.reads[(ch, payload)]
+                if not r["ended"]:
+                    r["ended"] = True
+                    self.push(t + float(self.rng.exponential(self.cfg.gap_s)), "start", ch)
+                    yield ("end", t, ch, payload)
+
+    def decide(self, ch, num, action, now):
+        r = self.reads[(ch, num)]
+        if action not in (NONTARGET, TARGET) or r["decided"] is not None:
+            return False
+        r["decided"] = action
+        self.decided.append(r)
+        if action == NONTARGET:  # unblock: the read ends shortly after the decision
+            new_end = now + self.cfg.unblock_s
+            if new_end < r["end"] and not r["ended"]:
+                r["end"] = new_end
+                self.push(new_end, "end", ch, num)
+        return True
+
+    def done(self, now):
+        nt = self.next_time()
+        return nt is None or nt > self.cfg.duration_s + 5 * self.chunk_s
+
+    def metrics(self, sim_s, lat, lat_dec, batch_sizes, busy):
+        lat = np.asarray(lat) * 1e3
+        lat_dec = np.asarray(lat_dec) * 1e3
+        truth = np.array([r["target"] for r in self.decided], bool)
+        call = np.array([r["decided"] == TARGET for r in self.decided], bool)
+
+        def pct(a, q):
+            return round(float(np.percentile(a, q)), 3) if a.size else None
+        return dict(
+            channels=self.cfg.n_channels, sim_seconds=round(sim_s, 2),
+            reads_started=len(self.reads), reads_decided=len(self.decided),
+            reads_decided_per_s=round(len(self.decided) / sim_s, 1),
+            chunks_scored=int(lat.size), batches=len(batch_sizes),
+            mean_batch=round(float(np.mean(batch_sizes)), 2) if batch_sizes else 0.0,
+            max_batch=int(max(batch_sizes)) if batch_sizes else 0,
+            latency_ms_p50=pct(lat, 50), latency_ms_p99=pct(lat, 99),
+            latency_ms_max=round(float(lat.max()), 3) if lat.size else None,
+            decision_latency_ms_p50=pct(lat_dec, 50), decision_latency_ms_p99=pct(lat_dec, 99),
+            busy_frac=round(busy / sim_s, 4) if busy is not None else None,
+            accuracy=round(float((truth == call).mean()), 4) if truth.size else None,
+            recall_target=round(float(call[truth].mean()), 4) if truth.any() else None,
+        )
+
+
+def _one_per_channel(ready):
+    ready.sort()
+    take, keep, seen = [], [], set()
+    for item in ready:
+        if item[1] in seen:
+            keep.append(item)
+        else:
+            seen.add(item[1])
+            take.append(item)
+    return take, keep
 
 
 def simulate(ru, source, cfg: SimConfig):
-    """Runs the flow cell for cfg.duration_s simulated seconds; returns metrics."""
-    rng = np.random.default_rng(cfg.seed)
-    L, fs = cfg.chunk_len, cfg.sample_rate
-    chunk_s = L / fs
-    events = []  # (time, seq, kind, channel, payload)
-    seq = 0
-
-    def push(t, kind, ch, payload=None):
-        nonlocal seq
-        heapq.heappush(events, (t, seq, kind, ch, payload))
-        seq += 1
-
-    reads = {}  # (channel, number) -> dict
-    number = np.zeros(cfg.n_channels, np.int64)
-
-    def start_read(ch, t0):
-        number[ch] += 1
-        n = max(L, int(rng.exponential(cfg.read_mean)))
-        target = bool(rng.random() < cfg.target_frac)
-        n_chunks = min(cfg.max_chunks, n // L)
-        r = dict(t0=t0, n=n, target=target, end=t0 + n / fs, decided=None, t_dec=None,
-                 sig=source.signal(n_chunks * L, target), n_chunks=n_chunks)
-        reads[(ch, int(number[ch]))] = r
-        for k in range(n_chunks):
-            push(t0 + (k + 1) * chunk_s, "chunk", ch, (int(number[ch]), k))
-        push(r["end"], "end", ch, int(number[ch]))
-
-    for ch in range(cfg.n_channels):
-        push(float(rng.exponential(cfg.gap_s)), "start", ch)
-
+    """Serving form, event by event (see the module docstring)."""
+    fc, L = FlowCell(source, cfg), cfg.chunk_len
     now = 0.0
     ready = []  # (arrival, channel, number, k)
     lat, lat_dec, batch_sizes, busy = [], [], [], 0.0
-    decided = []
     while True:
-        while events and events[0][0] <= now:
-            t, _, kind, ch, payload = heapq.heappop(events)
-            if kind == "start":
-                if t < cfg.duration_s:
-                    start_read(ch, t)
-            elif kind == "chunk":
-                num, k = payload
-                r = reads[(ch, num)]
-                if r["decided"] is None and not r.get("ended"):
-                    ready.append((t, ch, num, k))
-            elif kind == "end":
-                r = reads[(ch, payload)]
-                if not r.get("ended"):
-                    r["ended"] = True
-                    ru.release(ch)
-                    ready = [it for it in ready if (it[1], it[2]) != (ch, payload)]
-                    push(t + float(rng.exponential(cfg.gap_s)), "start", ch)
-        if not ready:
-            if not events or events[0][0] > cfg.duration_s + 5 * chunk_s:
-                break
-            now = max(now, events[0][0])
-            continue
-        # one chunk per channel, earliest first; the rest wait
-        ready.sort()
-        take, keep, seen = [], [], set()
-        for item in ready:
-            if item[1] in seen:
-                keep.append(item)
+        for ev in fc.pop_due(now):
+            if ev[0] == "chunk":
+                ready.append(ev[1:])
             else:
-                seen.add(item[1])
-                take.append(item)
-        ready = keep
-        chans = np.array([x[1] for x in take], np.int32)
-        nums = np.array([x[2] for x in take], np.int32)
-        sig = np.stack([reads[(x[1], x[2])]["sig"][x[3] * L:(x[3] + 1) * L] for x in take])
+                _, t, ch, num = ev
+                ru.release(ch)
+                ready = [it for it in ready if (it[1], it[2]) != (ch, num)]
+        if not ready:
+            if fc.done(now):
+                break
+            now = max(now, fc.next_time())
+            continue
+        take, ready = _one_per_channel(ready)
+        sig = np.stack([fc.reads[(x[1], x[2])]["sig"][x[3] * L:(x[3] + 1) * L] for x in take])
         t0 = time.perf_counter()
-        dec = ru.process(chans, nums, sig)
+        dec = ru.process([x[1] for x in take], [x[2] for x in take], sig)
         dt = time.perf_counter() - t0
         busy += dt
         now += dt
         batch_sizes.append(len(take))
         for x, d in zip(take, dec):
             lat.append(now - x[0])
-            r = reads[(x[1], x[2])]
-            a = int(d["action"])
-            if a in (1, 2) and r["decided"] is None:
-                r["decided"] = a
-                r["t_dec"] = now
+            if fc.decide(x[1], x[2], int(d["action"]), now):
                 lat_dec.append(now - x[0])
-                decided.append(r)
-                if a == 1:  # unblock: the read ends shortly after the decision
-                    new_end = now + cfg.unblock_s
-                    if new_end < r["end"]:
-                        r["end"] = new_end
-                        push(new_end, "end", x[1], x[2])
-            # ready chunks of a decided read are dropped
-        ready = [it for it in ready if reads[(it[1], it[2])]["decided"] is None]
-    lat = np.array(lat) * 1e3
-    lat_dec = np.array(lat_dec) * 1e3
-    truth = np.array([r["target"] for r in decided])
-    call = np.array([r["decided"] == 2 for r in decided])
-    sim_s = max(now, cfg.duration_s)
+        ready = [it for it in ready if fc.reads[(it[1], it[2])]["decided"] is None]
+    out = fc.metrics(max(now, cfg.duration_s), lat, lat_dec, batch_sizes, busy)
     st = ru.stats()
-    out = dict(
-        channels=cfg.n_channels, sim_seconds=round(sim_s, 2),
-        reads_started=len(reads), reads_decided=len(decided),
-        reads_decided_per_s=round(len(decided) / sim_s, 1),
-        chunks_scored=int(len(lat)), batches=len(batch_sizes),
-        mean_batch=round(float(np.mean(batch_sizes)) if batch_sizes else 0.0, 2),
-        max_batch=int(max(batch_sizes)) if batch_sizes else 0,
-        latency_ms_p50=round(float(np.percentile(lat, 50)), 3) if lat.size else None,
-        latency_ms_p99=round(float(np.percentile(lat, 99)), 3) if lat.size else None,
-        latency_ms_max=round(float(lat.max()), 3) if lat.size else None,
-        decision_latency_ms_p99=round(float(np.percentile(lat_dec, 99)), 3) if lat_dec.size else None,
-        busy_frac=round(busy / sim_s, 4),
-        gpu_cells_per_s_busy=round(st["cells"] / max(st["gpu_ms"] * 1e-3, 1e-9) / 1e12, 3),
-        accuracy=round(float((truth == call).mean()), 4) if truth.size else None,
-        recall_target=round(float(call[truth].mean()), 4) if truth.any() else None,
-    )
+    out["gpu_cells_per_s_busy"] = round(st["cells"] / max(busy, 1e-9) / 1e12, 3)
     return out
+
+
+def _take_decision(fc, d, now, lat, lat_dec):
+    a = int(d["action"])
+    if a > TARGET:  # NO_SLOT / STALE: answered without scoring
+        return
+    lat.append(d["latency_us"] * 1e-6)
+    if fc.decide(int(d["channel"]), int(d["number"]), a, now):
+        lat_dec.append(d["latency_us"] * 1e-6)
+
+
+def simulate_realtime(ru, source, cfg: SimConfig):
+    """Serving form in wall-clock time through the engine's serving loop."""
+    fc, L = FlowCell(source, cfg), cfg.chunk_len
+    ru.start()
+    t_start = time.perf_counter()
+    lat, lat_dec = [], []
+    outstanding = 0
+    try:
+        while True:
+            now = time.perf_counter() - t_start
+            for d in ru.poll(4096, timeout_ms=0):
+                outstanding -= 1
+                _take_decision(fc, d, now, lat, lat_dec)
+            for ev in fc.pop_due(now):
+                if ev[0] == "chunk":
+                    _, t, ch, num, k = ev
+                    ru.submit(ch, num, fc.reads[(ch, num)]["sig"][k * L:(k + 1) * L])
+                    outstanding += 1
+                else:
+                    ru.release(ev[2])
+            if fc.done(now) and outstanding == 0:
+                break
+            nt = fc.next_time()
+            wait = 0.002 if nt is None else min(0.002, max(0.0, nt - (time.perf_counter() - t_start)))
+            if outstanding:
+                got = ru.poll(4096, timeout_ms=int(wait * 1e3))
+                now = time.perf_counter() - t_start
+                for d in got:
+                    outstanding -= 1
+                    _take_decision(fc, d, now, lat, lat_dec)
+            elif wait > 0:
+                time.sleep(wait)
+    finally:
+        ru.stop()
+    sim_s = max(time.perf_counter() - t_start, cfg.duration_s)
+    st = ru.stats()
+    out = fc.metrics(sim_s, lat, lat_dec, [], None)
+    out["batches"] = st["batches"]
+    out["mean_batch"] = round(st["chunks"] / max(st["batches"], 1), 2)
+    out["gpu_kernel_s_sum"] = round(st["gpu_ms"] * 1e-3, 3)
+    return out
+
+
+def simulate_plain(ctx, source, cfg: SimConfig, threshold_per_row, precision="f16", norm=None):
+    """Plain batched form, event by event (see the module docstring)."""
+    fc, L = FlowCell(source, cfg), cfg.chunk_len
+    need = cfg.max_chunks
+    now = 0.0
+    ready = []  # (arrival of the last chunk, channel, number)
+    lat, lat_dec, batch_sizes, busy = [], [], [], 0.0
+    while True:
+        for ev in fc.pop_due(now):
+            if ev[0] == "chunk":
+                _, t, ch, num, k = ev
+                if k == need - 1:
+                    ready.append((t, ch, num))
+            else:
+                _, t, ch, num = ev
+                ready = [it for it in ready if (it[1], it[2]) != (ch, num)]
+        if not ready:
+            if fc.done(now):
+                break
+            now = max(now, fc.next_time())
+            continue
+        take, ready = ready, []
+        raws = [fc.reads[(x[1], x[2])]["sig"][:need * L] for x in take]
+        t0 = time.perf_counter()
+        best = _plain_scores(ctx, raws, precision, norm)
+        dt = time.perf_counter() - t0
+        busy += dt
+        now += dt
+        batch_sizes.append(len(take))
+        for x, s in zip(take, best):
+            a = NONTARGET if s >= threshold_per_row * need * L else TARGET
+            lat.append(now - x[0])
+            if fc.decide(x[1], x[2], a, now):
+                lat_dec.append(now - x[0])
+    return fc.metrics(max(now, cfg.duration_s), lat, lat_dec, batch_sizes, busy)

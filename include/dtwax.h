@@ -217,7 +217,9 @@ typedef struct {
   int64_t hbm_budget;    /* bytes of read state at most (0: max_reads slots) */
   float threshold_per_row; /* > 0: length-aware rule, the threshold after k chunks is
                               threshold_per_row * k * chunk_len (replaces threshold) */
-  int32_t reserved[3];
+  int32_t n_streams;     /* serving loop: batches in flight at once, each on its own HIP
+                            stream (1..8; 0: 2) */
+  int32_t reserved[2];
 } dtwax_ru_opts;
 
 typedef struct {
@@ -242,9 +244,10 @@ typedef struct {
  * lives). */
 int dtwax_ru_create(dtwax_ctx* ctx, const dtwax_ru_opts* opts, dtwax_ru** out);
 int dtwax_ru_destroy(dtwax_ru* ru);
-int dtwax_ru_info(const dtwax_ru* ru, int32_t* slots, int32_t* active, int32_t* segment,
+int dtwax_ru_info(dtwax_ru* ru, int32_t* slots, int32_t* active, int32_t* segment,
                   int64_t* state_bytes);
-int dtwax_ru_stats(const dtwax_ru* ru, int64_t* batches, int64_t* chunks, int64_t* cells,
+/* Batches, chunks scored, cells, and the sum of the sDTW kernels' durations. */
+int dtwax_ru_stats(dtwax_ru* ru, int64_t* batches, int64_t* chunks, int64_t* cells,
                    double* gpu_ms);
 /* Scores one batch: chunk i is chunk_len host samples at samples + offset.
  * At most one chunk per channel per call (EINVAL otherwise).  Writes one
@@ -253,10 +256,14 @@ int dtwax_ru_process(dtwax_ru* ru, const dtwax_ru_chunk* chunks, int32_t n, cons
                      dtwax_ru_decision* out);
 /* The read on `channel` ended: free its state. */
 int dtwax_ru_release(dtwax_ru* ru, int32_t channel);
-/* Serving loop: a worker thread scores, as one batch, every chunk submitted
- * while the previous batch ran (the earliest chunk per channel; later ones
- * wait for the next batch).  submit copies chunk_len samples; poll returns up
- * to cap decisions, waiting up to timeout_ms (-1: until one is available). */
+/* Serving loop (continuous batching): a worker thread launches, as one
+ * batch on a free stream, every chunk that has arrived (the earliest per
+ * channel, for channels with no chunk on the GPU), while earlier batches are
+ * still running; up to n_streams batches run on the device at once.
+ * submit copies chunk_len samples; poll returns up to cap decisions, waiting
+ * up to timeout_ms (-1: until one is available).  While the loop runs,
+ * dtwax_ru_process is refused and dtwax_ru_release is queued in order with
+ * the channel's chunks. */
 int dtwax_ru_start(dtwax_ru* ru);
 int dtwax_ru_submit(dtwax_ru* ru, int32_t channel, int32_t number, const float* samples);
 int dtwax_ru_poll(dtwax_ru* ru, dtwax_ru_decision* out, int32_t cap, int32_t* n_out,

@@ -174,18 +174,27 @@ def test_ru_decisions_slots_and_staleness(ctx):
         ru.close()
 
 
-def test_ru_serving_loop_matches_process(ctx):
+@pytest.mark.parametrize("n_streams,spaced", [(1, False), (2, False), (4, True)])
+def test_ru_serving_loop_matches_process(ctx, n_streams, spaced):
+    """The serving loop (continuous batching; with spaced submissions several
+    batches are on the device at once) gives each read the oracle's scores."""
+    import time
     L, n_chunks = 1600, 3
     w = _workload(seed=9, n_reads=5, chunk_len=L, n_chunks=n_chunks)
     ref = w.ref_levels.astype(np.float32)
     ctx.upload_reference(ref)
-    ru = D.ReadUntil(ctx, max_reads=8, chunk_len=L, max_chunks=n_chunks, precision="f16")
+    ru = D.ReadUntil(ctx, max_reads=8, chunk_len=L, max_chunks=n_chunks, precision="f16",
+                     n_streams=n_streams)
     try:
         ru.start()
-        # every chunk of every read up front: the loop must keep per-channel order
+        # chunks of every read in chunk order: the loop must keep per-channel order
         for k in range(n_chunks):
             for i, r in enumerate(w.reads):
                 ru.submit(i, 7, r[k * L:(k + 1) * L])
+                if spaced:
+                    time.sleep(0.0003)
+        with pytest.raises(D.DtwaxError):  # process() is refused while the loop runs
+            ru.process([99], [1], np.zeros((1, L), np.float32))
         got = []
         while len(got) < n_chunks * len(w.reads):
             d = ru.poll(64, timeout_ms=20000)
@@ -231,3 +240,30 @@ def test_ru_simulated_flow_cell(ctx):
     assert m["accuracy"] > 0.9
     assert m["chunks_scored"] == st["chunks"]
     assert 0 < m["latency_ms_p50"] <= m["latency_ms_p99"]
+
+
+def test_ru_realtime_and_plain_forms(ctx):
+    """The serving loop driven in wall-clock time and the plain batched form
+    on the same small flow cell: both decide, and the serving form decides
+    more reads per second."""
+    from dtwax_amd import ru_sim
+    rng = np.random.default_rng(31)
+    means, _ = synth.make_model(rng)
+    seq, levels = synth.build_reference(rng, 20_000, means)
+    ctx.upload_reference(levels)
+    src = ru_sim.ReadSource(seq, means, np.random.default_rng(32))
+    thr, _ = ru_sim.calibrate_threshold_per_row(ctx, src, 1600, 4, n_each=16)
+    pthr, _ = ru_sim.calibrate_plain_threshold(ctx, src, 6400, n_each=16)
+    cfg = ru_sim.SimConfig(n_channels=48, duration_s=4.0, target_frac=0.3, seed=4)
+    ru = D.ReadUntil(ctx, max_reads=48, chunk_len=1600, max_chunks=4, precision="f16",
+                     threshold_per_row=thr, n_streams=2)
+    try:
+        rt = ru_sim.simulate_realtime(ru, src, cfg)
+    finally:
+        ru.close()
+    plain = ru_sim.simulate_plain(ctx, src, cfg, pthr)
+    print("realtime", rt)
+    print("plain", plain)
+    assert rt["reads_decided"] > 30 and rt["accuracy"] > 0.85
+    assert plain["reads_decided"] > 10
+    assert rt["reads_decided_per_s"] > plain["reads_decided_per_s"]
