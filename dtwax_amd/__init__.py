@@ -34,6 +34,8 @@ class ScoreOpts(C.Structure):
 
 
 RESULT_DTYPE = np.dtype([("score", np.float32, 2), ("argmin", np.int32, 2)])
+TUNE_DTYPE = np.dtype([("segment", np.int32), ("ref_chunk", np.int32), ("ms", np.float32),
+                       ("gcups", np.float32)])
 
 
 def lib():
@@ -75,6 +77,8 @@ def lib():
             "dtwax_plan_kernel_ms": ([vp, C.POINTER(f32)], C.c_int),
             "dtwax_distances": ([vp, vp, vp, vp, i32, C.c_uint32, vp, vp, i32], C.c_int),
             "dtwax_selftest_lanes": ([vp], C.c_int),
+            "dtwax_autotune": ([vp, vp, vp, vp, i32, vp, i32, i64, C.c_char_p, vp, vp, i32,
+                                C.POINTER(i32)], C.c_int),
             "dtwax_ru_create": ([vp, vp, C.POINTER(vp)], C.c_int),
             "dtwax_ru_destroy": ([vp], C.c_int),
             "dtwax_ru_info": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)],
@@ -106,7 +110,7 @@ def exported_symbols():
         "dtwax_reference_upload", "dtwax_squiggle_build", "dtwax_normalize", "dtwax_score",
         "dtwax_plan_create", "dtwax_plan_destroy", "dtwax_plan_info", "dtwax_score_plan",
         "dtwax_plan_set_timing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
-        "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
+        "dtwax_autotune", "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
         "dtwax_ru_release", "dtwax_ru_start", "dtwax_ru_submit", "dtwax_ru_poll", "dtwax_ru_stop",
     ]
 
@@ -270,6 +274,22 @@ class Context:
 
     def plan(self, lens, **kw):
         return Plan(self, lens, **kw)
+
+    def autotune(self, d_queries, d_offsets, lens, sample_reads=0, sample_cols=0, cache_path=None,
+                 **kw):
+        """Times candidate (segment, ref_chunk) plans on a sample of the
+        workload (device queries/offsets, host lengths).  Returns
+        (dict(segment=, ref_chunk=), table of timed candidates)."""
+        lens = np.ascontiguousarray(lens, np.int32)
+        base = make_opts(**kw)
+        best = ScoreOpts()
+        table = np.zeros(64, TUNE_DTYPE)
+        n = C.c_int32()
+        _check(lib().dtwax_autotune(self.handle, d_queries.ptr, d_offsets.ptr, lens.ctypes.data,
+                                    len(lens), C.byref(base), sample_reads, sample_cols,
+                                    None if cache_path is None else str(cache_path).encode(),
+                                    C.byref(best), table.ctypes.data, len(table), C.byref(n)))
+        return dict(segment=best.segment, ref_chunk=best.ref_chunk), table[: n.value]
 
     def distances(self, d_ref_tiled, d_query_h2, d_dists, num_entries, compat_quirks=0,
                   stream=None):

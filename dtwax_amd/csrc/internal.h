@@ -67,7 +67,34 @@ struct dtwax_ctx {
   int32_t compat_plan_n = -1;
 };
 
+// A plan: work items (read x reference chunk) in decreasing cost order and
+// the buffers one launch needs.
+struct dtwax_plan {
+  dtwax_ctx* ctx = nullptr;
+  int32_t n_reads = 0;
+  int precision = DTWAX_F16;
+  int S = 32;
+  int nstr = 2;
+  int32_t chunk = 0;
+  bool inj = false;
+  int grid = 0;
+  int32_t n_items = 0;
+  int64_t cells = 0, computed = 0;
+  float renorm_k = 0.f;
+  dtwax::WorkItem* d_items = nullptr;
+  int32_t* d_counter = nullptr;
+  unsigned long long* d_keys = nullptr;
+  void* d_scratch = nullptr;
+  int64_t scratch_stride = 0;
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed_once = false;
+};
+
 namespace dtwax {
 int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st);
 int occupancy(int precision, int S, int nstr, bool inj);
+// dtwax_plan_create over reference columns [0, col_limit) only
+int plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, const dtwax_score_opts* opts,
+               int64_t col_limit, dtwax_plan** out);
 }  // namespace dtwax

@@ -73,8 +73,8 @@ typedef struct {
   int32_t ref_chunk;     /* reference columns per work item; 0 = automatic (whole strand when
                             it fits); chunks overlap by (query length - 1) columns, exact for
                             the no-deletion recurrence */
-  float renorm_drift;    /* DTWAX_F16_RENORM: per-row drift subtracted at lane boundaries;
-                            <= 0 = default 0.1 */
+  float renorm_drift;    /* DTWAX_F16_RENORM: initial per-row drift subtracted at lane
+                            boundaries (adapted band to band); <= 0 = default 0.01 */
   int32_t compat_quirks; /* 1: reproduce the reference kernel's band-seam and tile-seam
                             deviations (DTW.cu:179-180, 315, 385-388, 442); needs the
                             reference geometry (see dtwax_distances) */
@@ -162,6 +162,34 @@ int dtwax_score_plan(dtwax_plan* plan, const float* d_queries, const int64_t* d_
  * dtwax_plan_kernel_ms then waits for and returns the last launch's time. */
 int dtwax_plan_set_timing(dtwax_plan* plan, int32_t enable);
 int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
+
+/* ---- autotuner -----------------------------------------------------------
+ * Picks segment and ref_chunk for a workload (no counterpart in the
+ * reference, whose geometry is fixed at compile time, src/include/common.hpp:
+ * 60-98): times the sDTW kernel for every candidate (segment in {8, 16, 25,
+ * 32} ({8, 16} for DTWAX_F16_RENORM) x ref_chunk in {1024, 4096, 16384,
+ * 65536, 262144}; a nonzero base->segment or base->ref_chunk fixes that
+ * one) on sample_reads reads (<= 0: 512) spread over the sorted lengths,
+ * against the first sample_cols reference columns (<= 0: 2^20), and writes
+ * the fastest per counted cell into *best (base's other fields kept).
+ * d_queries/d_offsets: the workload's device queries (as for dtwax_score);
+ * h_lens: host lengths.  The choice is cached per device, options, strand
+ * count, reference-size and length-distribution key, in memory and, if
+ * cache_path is not NULL, appended to that text file (read on later calls);
+ * a cached call times nothing and leaves *n_table = 0.  table (optional)
+ * receives up to table_cap timed candidates.  Synchronous. */
+typedef struct {
+  int32_t segment;
+  int32_t ref_chunk;
+  float ms;    /* kernel time on the sample */
+  float gcups; /* counted cells / kernel time */
+} dtwax_tune_entry;
+
+int dtwax_autotune(dtwax_ctx* ctx, const float* d_queries, const int64_t* d_offsets,
+                   const int32_t* h_lens, int32_t n_reads, const dtwax_score_opts* base,
+                   int32_t sample_reads, int64_t sample_cols, const char* cache_path,
+                   dtwax_score_opts* best, dtwax_tune_entry* table, int32_t table_cap,
+                   int32_t* n_table);
 
 /* Mirror of FullDTW::distances (src/include/DTW.hpp:285-295) for the
  * reference's fixed geometry (QUERY_LEN 2048, PREFIX_LEN 1024, SEGMENT 26,

@@ -41,13 +41,18 @@ void orc_sdtw_f64(const double* q, int64_t Q, const double* s, int64_t N,
   for (int64_t c = 0; c < N; c++) a[c] = init_row ? init_row[c] : 0.0;
   for (int64_t r = 0; r < Q; r++) {
     const double qr = q[r];
-    double left = INFINITY; /* D[r-1][c-1], column -1 = +inf */
-    for (int64_t c = 0; c < N; c++) {
+    /* column 0: diag is column -1 = +inf, so min(top, diag) = top */
+    if (N > 0) {
+      double t = s[0] - qr;
+      b[0] = t * t + a[0];
+    }
+    /* no dependency along the row (no reference deletion): vectorisable */
+#pragma omp simd
+    for (int64_t c = 1; c < N; c++) {
       double t = s[c] - qr;
-      double top = a[c];
+      double top = a[c], left = a[c - 1];
       double m = top < left ? top : left;
       b[c] = t * t + m;
-      left = top;
     }
     double* tmp = a; a = b; b = tmp;
   }

@@ -183,3 +183,27 @@ def test_c2_subset_decisions(ctx):
     # the synthetic targets and non-targets separate at the best threshold
     acc = max(np.mean((best < th) == w.is_target) for th in np.unique(best))
     assert acc >= 0.9, acc
+
+
+def test_autotune_choice_cached_and_exact(ctx, tmp_path):
+    """The autotuner times every candidate on a sample, caches its choice
+    (memory and file), and the chosen plan scores exactly like any other."""
+    rng = np.random.default_rng(77)
+    ref = rng.standard_normal((2, 20000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    lens = rng.integers(100, 3000, 40)
+    reads = [rng.standard_normal(int(n)).astype(np.float32) for n in lens]
+    flat, offs, lns = D.pack_reads(reads)
+    d_q, d_off = D.DeviceArray.from_host(flat), D.DeviceArray.from_host(offs)
+    cache = tmp_path / "tune.txt"
+    best, table = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, cache_path=cache)
+    assert len(table) >= 8
+    assert best["segment"] in (8, 16, 25, 32) and best["ref_chunk"] in (1024, 4096, 16384, 65536, 262144)
+    top = table[np.argmax(table["gcups"])]
+    assert (top["segment"], top["ref_chunk"]) == (best["segment"], best["ref_chunk"])
+    assert cache.read_text().split()[-2:] == [str(best["segment"]), str(best["ref_chunk"])]
+    again, t2 = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, cache_path=cache)
+    assert again == best and len(t2) == 0
+    sc, am = ctx.score(reads, precision="f16", **best)
+    sc0, am0 = ctx.score(reads, precision="f16", segment=32, ref_chunk=0)
+    assert np.array_equal(sc, sc0) and np.array_equal(am, am0)

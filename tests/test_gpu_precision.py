@@ -1,0 +1,109 @@
+"""Precision sweep: each arithmetic form against the FP64 oracle.
+
+Workload: reads of the 48.5 kb configuration's generator (50% target),
+truncated to each query length Q, median/MAD normalised on the GPU, scored
+by the GPU in binary16 per-op (the reference's half2 arithmetic), binary16
+with drift/minimum renormalisation, and binary32; the same normalised
+queries are scored by the FP64 oracle.  Reported per form and Q: max/mean
+absolute and relative error of min(fwd, rev) and of each strand, and the
+number of decisions that flip against FP64 at a fixed, length-scaled
+threshold (per-row threshold = midpoint of the FP64 classes at Q = 2000,
+times Q).
+
+The default test runs a 48-read subset at Q = 1000, 2000 and 8000 and checks the
+expected ordering of the forms.  With DTWAX_FULL_SWEEP=1 the full sweep
+(1024 reads x Q in {1000, 2000, 4000, 8000}) runs and writes its report to
+DTWAX_SWEEP_OUT (default precision_sweep.json).
+"""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+import dtwax_amd as D
+import oracle as O
+from dtwax_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ("f16", "f16_renorm", "f32")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = D.Context(0)
+    yield c
+    c.close()
+
+
+def _cut(per_row, truth):
+    t_hi, n_lo = per_row[truth].max(), per_row[~truth].min()
+    if t_hi < n_lo:
+        return 0.5 * (t_hi + n_lo)
+    cands = np.unique(per_row)
+    acc = [((per_row[truth] < c).mean() + (per_row[~truth] >= c).mean()) for c in cands]
+    return float(cands[int(np.argmax(acc))])
+
+
+def sweep(ctx, n_reads, lengths, seed=3, threads=0):
+    w = synth.config_workload(seed=seed, ref_bp=48_500, n_reads=n_reads, raw_len=max(lengths),
+                              target_frac=0.5)
+    ref = w.ref_levels.astype(np.float32)
+    ctx.upload_reference(ref)
+    truth = w.is_target
+    per_q, thr_row = {}, None
+    for Q in sorted(lengths, key=lambda q: (q != 2000, q)):  # Q = 2000 first: fixes the threshold
+        normed, _ = ctx.normalize([r[:Q] for r in w.reads], mode=D.NORM_MEDMAD)
+        t0 = time.time()
+        f64, _ = O.batch_f64(normed, ref.astype(np.float64), n_threads=threads)
+        t_cpu = time.time() - t0
+        b64 = f64.min(axis=1)
+        if thr_row is None:
+            thr_row = _cut(b64 / Q, truth)
+        thr = thr_row * Q
+        dec64 = b64 < thr
+        row = {"fp64_cpu_s": round(t_cpu, 2), "fp64_accuracy": float((dec64 == truth).mean())}
+        for form in FORMS:
+            sc, _ = ctx.score(normed, precision=form)
+            sc = sc.astype(np.float64)
+            best = sc.min(axis=1)
+            ae = np.abs(best - b64)
+            re = ae / np.maximum(b64, 1e-12)
+            ae_s = np.abs(sc - f64)
+            re_s = ae_s / np.maximum(f64, 1e-12)
+            row[form] = {
+                "max_abs": float(ae.max()), "mean_abs": float(ae.mean()),
+                "max_rel": float(re.max()), "mean_rel": float(re.mean()),
+                "strand_max_rel": float(re_s.max()), "strand_mean_rel": float(re_s.mean()),
+                "flips": int(((best < thr) != dec64).sum()),
+                "accuracy": float(((best < thr) == truth).mean()),
+            }
+        per_q[int(Q)] = row
+    return {"reads": n_reads, "ref_cols_per_strand": int(ref.shape[1]),
+            "threshold_per_row": float(thr_row), "lengths": {q: per_q[q] for q in sorted(per_q)}}
+
+
+def test_precision_subset(ctx):
+    rep = sweep(ctx, 48, (1000, 2000, 8000))
+    print(json.dumps(rep))
+    for Q, row in rep["lengths"].items():
+        f16, rn, f32 = row["f16"], row["f16_renorm"], row["f32"]
+        assert f32["max_rel"] < 1e-4, (Q, f32)
+        assert f32["flips"] == 0
+        assert rn["max_rel"] < 0.02 and rn["mean_rel"] < 0.005, (Q, rn)
+        assert rn["mean_rel"] <= f16["mean_rel"], (Q, rn, f16)
+        assert rn["flips"] <= f16["flips"] + 1
+    # binary16 cost accumulation drifts with the query length
+    assert rep["lengths"][8000]["f16"]["mean_rel"] > rep["lengths"][1000]["f16"]["mean_rel"]
+
+
+@pytest.mark.skipif(not os.environ.get("DTWAX_FULL_SWEEP"), reason="full sweep: set DTWAX_FULL_SWEEP=1")
+def test_precision_full_sweep(ctx):
+    t0 = time.time()
+    rep = sweep(ctx, 1024, (1000, 2000, 4000, 8000))
+    rep["wall_s"] = round(time.time() - t0, 1)
+    with open(os.environ.get("DTWAX_SWEEP_OUT", "precision_sweep.json"), "w") as f:
+        json.dump(rep, f, indent=1)
+    print(json.dumps(rep))
