@@ -76,8 +76,7 @@ void dtwax_destroy(dtwax_ctx* c) {
   (void)hipDeviceSynchronize();
   free_reference(c);
   if (c->compat_plan) dtwax_plan_destroy(c->compat_plan);
-  for (void* p : {(void*)c->compat_ref, (void*)c->compat_q, (void*)c->compat_off, (void*)c->compat_len,
-                  (void*)c->compat_res})
+  for (void* p : {(void*)c->compat_q, (void*)c->compat_off, (void*)c->compat_len, (void*)c->compat_res})
     if (p) (void)hipFree(p);
   delete c;
 }
@@ -491,19 +490,19 @@ int dtwax_distances(dtwax_ctx* c, const void* ref, const void* query, void* dist
   (void)device_last_row;  // band scratch is internal
   if (!c || !ref || !query || !dists) return fail(DTWAX_EINVAL, "NULL argument");
   if (num_entries < 0) return fail(DTWAX_EINVAL, "num_entries < 0");
-  if (compat_quirks) return fail(DTWAX_EINVAL, "compat_quirks=1 is not built into this library yet");
+  if (compat_quirks != 0 && compat_quirks != 1) return fail(DTWAX_EINVAL, "compat_quirks must be 0 or 1");
   if (num_entries == 0) return DTWAX_OK;
   HIP_TRY(hipSetDevice(c->device));
   hipStream_t st = S_(stream);
   const int32_t tile = kRefSeg * kRefWarp;
   const int32_t n_tiles = kRefLen / tile;  // 57: columns beyond 57*832 are never scored
   const int32_t n_cols = n_tiles * tile;
-  // reference: untile into a private padded buffer and make it current
-  if (c->n_cols != n_cols || c->n_strands != 2 || !c->compat_ref) {
+  // reference: untile into the context's padded buffer (it becomes the current
+  // reference; the stored one is replaced, as the caller passes it every call)
+  if (c->n_cols != n_cols || c->n_strands != 2 || !c->ref16_alloc) {
     std::vector<float> zeros((size_t)2 * n_cols, 0.0f);
     int rc = dtwax_reference_upload(c, zeros.data(), 2, n_cols);
     if (rc) return rc;
-    c->compat_ref = c->ref16_alloc;
   }
   HIP_TRY(launch_untile_reference((const uint32_t*)ref, kRefLen, kRefSeg, kRefWarp, n_tiles,
                                   (uint32_t*)c->ref16_alloc + COL_PADL, st));
@@ -527,6 +526,12 @@ int dtwax_distances(dtwax_ctx* c, const void* ref, const void* query, void* dist
     c->compat_cap = num_entries;
   }
   HIP_TRY(launch_half2_queries_to_float((const uint32_t*)query, nq, c->compat_q, st));
+  if (compat_quirks) {
+    HIP_TRY(launch_compat_quirks((const uint32_t*)c->ref16_alloc + COL_PADL, c->compat_q, num_entries, n_tiles,
+                                 c->compat_res, st));
+    HIP_TRY(launch_results_to_half2(c->compat_res, num_entries, (uint32_t*)dists, st));
+    return DTWAX_OK;
+  }
   if (!c->compat_plan || c->compat_plan_n != num_entries) {
     if (c->compat_plan) dtwax_plan_destroy(c->compat_plan);
     c->compat_plan = nullptr;

@@ -19,8 +19,21 @@
 // reference, any query length, medmad, other precisions).
 // Exit status is non-zero on any error (the reference exits on the first
 // CUDA error and on a missing VBZ plugin).
+//
+//   dtwax_main <kmer_model> <ref.fasta> [pipe] [options]
+//
+// The reference's Read Until driver (src/ru_main.cu:34-326, same program
+// name, chosen here when the first argument is not a directory): reads
+// batches of READS x QUERY_LEN bytes (10 x 2048) from the FIFO `pipe` in the
+// working directory (generate_load_squiggle.hpp:90-130; each byte is one
+// unsigned sample), normalises, scores and prints one row per query without
+// a header: j, j, QUERY_LEN, REF_LEN, fwd, rev.  The reference loops
+// forever; here the loop ends when the writer closes the FIFO, unless
+// --keep-open (then the FIFO is reopened for the next writer).
 
 #include <sys/stat.h>
+
+#include <cstdint>
 
 #include <algorithm>
 #include <chrono>
@@ -57,7 +70,9 @@ struct Options {
 void usage() {
   std::cerr << "usage: dtwax_main <fast5_folder/> <kmer_model> <ref.fasta> [--native] [--query-len N]\n"
                "       [--adapter-len N] [--precision f16|f16_fused|f16_renorm|f32] [--norm meanstd|medmad]\n"
-               "       [--fp32-build] [--device N] [--dump-reads]\n";
+               "       [--fp32-build] [--device N] [--dump-reads]\n"
+               "   or: dtwax_main <kmer_model> <ref.fasta> [pipe] [--reads N] [--query-len N] [--native]\n"
+               "       [--precision ...] [--norm ...] [--device N] [--keep-open]   (Read Until FIFO loop)\n";
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -192,9 +207,129 @@ bool load_model(const std::string& path, int& k, std::vector<float>& means) {
     }                                                                             \
   } while (0)
 
+bool is_directory(const char* p) {
+  struct stat st;
+  return stat(p, &st) == 0 && S_ISDIR(st.st_mode);
+}
+
+// Read Until FIFO loop (src/ru_main.cu:172-326).
+int ru_main(int argc, char** argv) {
+  std::string model = argv[1], fasta = argv[2], pipe = "pipe";
+  int reads = 10, query_len = 2048, device = 0, precision = DTWAX_F16, norm = DTWAX_NORM_MEANSTD;
+  bool native = false, keep_open = false;
+  for (int i = 3; i < argc; i++) {
+    std::string a = argv[i];
+    auto next = [&]() -> const char* {
+      if (i + 1 >= argc) {
+        std::cerr << "missing value for " << a << "\n";
+        std::exit(2);
+      }
+      return argv[++i];
+    };
+    if (i == 3 && a.rfind("--", 0) != 0) pipe = a;
+    else if (a == "--reads") reads = std::atoi(next());
+    else if (a == "--query-len") query_len = std::atoi(next());
+    else if (a == "--device") device = std::atoi(next());
+    else if (a == "--native") native = true;
+    else if (a == "--keep-open") keep_open = true;
+    else if (a == "--precision") {
+      std::string p = next();
+      if (p == "f16") precision = DTWAX_F16;
+      else if (p == "f16_fused") precision = DTWAX_F16_FUSED;
+      else if (p == "f16_renorm") precision = DTWAX_F16_RENORM;
+      else if (p == "f32") precision = DTWAX_F32;
+      else { usage(); return 2; }
+    } else if (a == "--norm") {
+      std::string p = next();
+      if (p == "meanstd") norm = DTWAX_NORM_MEANSTD;
+      else if (p == "medmad") norm = DTWAX_NORM_MEDMAD;
+      else { usage(); return 2; }
+    } else {
+      std::cerr << "unknown option " << a << "\n";
+      usage();
+      return 2;
+    }
+  }
+  if (reads < 1 || query_len < 1) {
+    usage();
+    return 2;
+  }
+  // reference: as the offline driver (REF_LEN 48128 k-mers, 57 x 832 scored)
+  std::string seq;
+  int k = 6;
+  std::vector<float> means;
+  if (!load_fasta(fasta, seq) || !load_model(model, k, means)) return 1;
+  const int64_t n_kmers = (int64_t)seq.size() - k + 1;
+  const int64_t ref_len = native ? n_kmers : 48128;
+  if (ref_len < 1) {
+    std::cerr << "reference shorter than one k-mer" << std::endl;
+    return 1;
+  }
+  std::vector<float> levels((size_t)(2 * ref_len));
+  int64_t skipped = 0;
+  CHECK(dtwax_squiggle_build(seq.data(), (int64_t)seq.size(), k, means.data(), ref_len, levels.data(), &skipped));
+  const int64_t n_cols = native ? ref_len : std::min<int64_t>(47424, ref_len);
+  dtwax_ctx* ctx = nullptr;
+  CHECK(dtwax_create(device, &ctx));
+  {
+    std::vector<float> two((size_t)(2 * n_cols));
+    std::copy(levels.begin(), levels.begin() + n_cols, two.begin());
+    std::copy(levels.begin() + ref_len, levels.begin() + ref_len + n_cols, two.begin() + n_cols);
+    CHECK(dtwax_reference_upload(ctx, two.data(), 2, n_cols));
+  }
+  const size_t batch = (size_t)reads * (size_t)query_len;
+  std::vector<int64_t> offs((size_t)reads);
+  std::vector<int32_t> lens((size_t)reads, query_len);
+  for (int i = 0; i < reads; i++) offs[(size_t)i] = (int64_t)i * query_len;
+  void *d_raw = nullptr, *d_norm = nullptr, *d_off = nullptr, *d_len = nullptr, *d_out = nullptr;
+  CHECK(dtwax_malloc(&d_raw, batch * 4));
+  CHECK(dtwax_malloc(&d_norm, batch * 4));
+  CHECK(dtwax_malloc(&d_off, offs.size() * 8));
+  CHECK(dtwax_malloc(&d_len, lens.size() * 4));
+  CHECK(dtwax_malloc(&d_out, (size_t)reads * sizeof(dtwax_result)));
+  CHECK(dtwax_memcpy_h2d(d_off, offs.data(), offs.size() * 8, nullptr));
+  CHECK(dtwax_memcpy_h2d(d_len, lens.data(), lens.size() * 4, nullptr));
+  dtwax_score_opts opt{};
+  opt.precision = precision;
+  dtwax_plan* plan = nullptr;
+  CHECK(dtwax_plan_create(ctx, lens.data(), reads, &opt, &plan));
+  std::vector<unsigned char> bytes(batch);
+  std::vector<float> raw(batch);
+  std::vector<dtwax_result> res((size_t)reads);
+  for (;;) {
+    std::FILE* f = std::fopen(pipe.c_str(), "rb");  // blocks until a writer opens the FIFO
+    if (!f) {
+      std::cerr << "Error opening '" << pipe << "'" << std::endl;
+      return 1;
+    }
+    for (;;) {
+      size_t got = std::fread(bytes.data(), 1, batch, f);
+      if (got < batch) break;  // writer closed (a partial batch is dropped)
+      for (size_t i = 0; i < batch; i++) raw[i] = (float)bytes[i];
+      CHECK(dtwax_memcpy_h2d(d_raw, raw.data(), batch * 4, nullptr));
+      CHECK(dtwax_normalize(ctx, (const float*)d_raw, (const int64_t*)d_off, (const int32_t*)d_len, reads, norm,
+                            1e-4, (float*)d_norm, nullptr, nullptr));
+      CHECK(dtwax_score_plan(plan, (const float*)d_norm, (const int64_t*)d_off, (const int32_t*)d_len,
+                             (dtwax_result*)d_out, nullptr));
+      CHECK(dtwax_memcpy_d2h(res.data(), d_out, res.size() * sizeof(dtwax_result), nullptr));
+      for (int j = 0; j < reads; j++)
+        std::cout << j << "\t" << j << "\t" << query_len << "\t" << ref_len << "\t" << res[(size_t)j].score[0]
+                  << "\t" << res[(size_t)j].score[1] << "\n";
+      std::cout << std::flush;
+    }
+    std::fclose(f);
+    if (!keep_open) break;
+  }
+  dtwax_plan_destroy(plan);
+  for (void* p : {d_raw, d_norm, d_off, d_len, d_out}) dtwax_free(p);
+  dtwax_destroy(ctx);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+  if (argc >= 3 && !is_directory(argv[1])) return ru_main(argc, argv);
   Options o;
   if (!parse(argc, argv, o)) {
     usage();

@@ -31,6 +31,105 @@ __global__ void q_h2_to_f32_kernel(const uint32_t* __restrict__ q, int64_t n, fl
   out[i] = (float)__builtin_bit_cast(_Float16, lo);
 }
 
+// The reference kernel's own deviations from the clean recurrence (FP16
+// build, QUERY_LEN 2048, PREFIX_LEN 1024, SEGMENT 26, WARP 32; tiles of 832
+// columns), reproduced on top of the clean binary16 recurrence:
+//   Q1  global row 1024 (first row of the second 1024-row band) takes +inf as
+//       its diagonal at c = 832k (k >= 1) and c = 832k + 26 (k >= 0): lane 0's
+//       diagonal is reset at every tile start (DTW.cu:315, 442) and lane 1's
+//       first diagonal is consumed before it is filled (DTW.cu:179-180,
+//       238-240);
+//   Q2  at c = 832k (k >= 2) rows 962..992 of each band take D[r + 30][c - 1]
+//       as their diagonal: middle tiles flush the last column to entry
+//       wave - 63 instead of wave - 32 (DTW.cu:385-388), so entries 961..991
+//       hold rows 992..1022.
+// One wave per read, 64 lanes x 32 rows = 2048 rows, lane L at column
+// 832t + T - L at step T of tile t.  The wave drains at every tile end like
+// the reference, so at a tile's first column the previous tile's last column
+// of every row is still in the lanes' registers (the Q2 source rows live in
+// lanes 31 and 63, which start the tile after lanes 30 and 62 use them).
+// Slow by design (a parity mode): inactive lanes keep their registers
+// through selects and the boundary values move by ds_bpermute.
+typedef _Float16 half_t;
+typedef half_t h2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2 shfl_h2(h2 v, int src) {
+  return __builtin_bit_cast(h2, __shfl(__builtin_bit_cast(int, v), src, 64));
+}
+
+__global__ __launch_bounds__(64) void sdtw_compat_quirks_kernel(const uint32_t* __restrict__ ref_h2,
+                                                                const float* __restrict__ queries,
+                                                                int32_t n_tiles, DeviceResult* __restrict__ out) {
+  constexpr int S = 32, TILE = 832, PRE = 1024;
+  const int lane = threadIdx.x;
+  const float* q = queries + (int64_t)blockIdx.x * 2048;
+  const h2 INF = (h2){(half_t)__builtin_inff(), (half_t)__builtin_inff()};
+  const h2 ZERO = (h2){(half_t)0.0f, (half_t)0.0f};
+  const h2 ONE = (h2){(half_t)1.0f, (half_t)1.0f};
+  h2 Qn[S], D[S], ov[S];
+  for (int i = 0; i < S; i++) {
+    const half_t v = (half_t)q[lane * S + i];
+    Qn[i] = (h2){(half_t)-v, (half_t)-v};
+    D[i] = INF;  // column -1
+    ov[i] = INF;
+  }
+  h2 tin = INF, dg = INF;
+  h2 best = INF;
+  int32_t posf = -1, posr = -1;
+  const int row0 = lane * S;
+  for (int32_t t = 0; t < n_tiles; t++) {
+    // Q2 sources: lanes 30 and 62 take rows 992.. / 2016.. of the previous
+    // tile's last column from lanes 31 and 63
+    for (int i = 2; i < S; i++) ov[i] = shfl_h2(D[i - 2], lane + 1 < 64 ? lane + 1 : lane);
+    const bool q2_tile = t >= 2;
+    for (int32_t T = 0; T < TILE + 63; T++) {
+      const int32_t cl = T - lane;
+      const bool active = cl >= 0 && cl < TILE;
+      const int32_t c = t * TILE + (active ? cl : 0);
+      const h2 s = __builtin_bit_cast(h2, ref_h2[c]);
+      h2 tp = (lane == 0) ? ZERO : tin;
+      h2 dd = (lane == 0) ? ZERO : dg;
+      if (lane == 32 && ((cl == 0 && t >= 1) || cl == 26)) dd = INF;  // Q1 (row 1024)
+      const bool q2_col = q2_tile && cl == 0;
+      if (q2_col && (lane == 31 || lane == 63)) dd = D[30];          // Q2 rows 992 / 2016
+      for (int i = 0; i < S; i++) {
+        const h2 tt = __builtin_elementwise_fma(s, ONE, Qn[i]);
+        const h2 c2 = tt * tt;
+        h2 m = __builtin_elementwise_min(tp, dd);
+        const h2 v = c2 + m;
+        const bool q2_row = q2_col && (lane == 30 || lane == 62) && i >= 1 && i + 1 < S;
+        dd = q2_row ? ov[i + 1] : D[i];  // diagonal of row i + 1
+        D[i] = active ? v : D[i];
+        tp = active ? v : tp;
+      }
+      if (lane == 63 && active) {
+        const h2 last = D[S - 1];
+        if (last.x < best.x) { best.x = last.x; posf = c; }
+        if (last.y < best.y) { best.y = last.y; posr = c; }
+      }
+      // boundary: lane L's next top input is lane L-1's last row now
+      dg = tin;
+      tin = shfl_h2(D[S - 1], lane > 0 ? lane - 1 : 0);
+      (void)row0;
+    }
+  }
+  if (lane == 63) {
+    DeviceResult r;
+    r.score[0] = (float)best.x;
+    r.score[1] = (float)best.y;
+    r.argmin[0] = posf;
+    r.argmin[1] = posr;
+    out[blockIdx.x] = r;
+  }
+}
+
+hipError_t launch_compat_quirks(const uint32_t* ref_h2, const float* queries, int32_t n_reads, int32_t n_tiles,
+                                DeviceResult* out, hipStream_t st) {
+  if (n_reads <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sdtw_compat_quirks_kernel, dim3(n_reads), dim3(64), 0, st, ref_h2, queries, n_tiles, out);
+  return hipGetLastError();
+}
+
 __global__ void res_to_h2_kernel(const DeviceResult* __restrict__ r, int32_t n, uint32_t* __restrict__ d) {
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

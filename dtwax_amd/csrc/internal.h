@@ -57,7 +57,6 @@ struct dtwax_ctx {
   void* ref16_alloc = nullptr;  // h2 per column, COL_PADL + n + COL_PADR
   void* ref32_alloc = nullptr;  // n_strands floats per column, same padding
   // compat entry point buffers
-  void* compat_ref = nullptr;
   float* compat_q = nullptr;
   int64_t* compat_off = nullptr;
   int32_t* compat_len = nullptr;

@@ -85,5 +85,9 @@ hipError_t launch_untile_reference(const uint32_t* tiled, int32_t ref_len, int32
                                    int32_t n_tiles, uint32_t* untiled_h2, hipStream_t st);
 hipError_t launch_half2_queries_to_float(const uint32_t* q_h2, int64_t n, float* out, hipStream_t st);
 hipError_t launch_results_to_half2(const DeviceResult* res, int32_t n, uint32_t* dists, hipStream_t st);
+// binary16 scores with the reference kernel's seam deviations (QUERY_LEN 2048,
+// PREFIX_LEN 1024, tiles of 832 columns): ref_h2 = flat {fwd, rev} columns
+hipError_t launch_compat_quirks(const uint32_t* ref_h2, const float* queries, int32_t n_reads, int32_t n_tiles,
+                                DeviceResult* out, hipStream_t st);
 
 }  // namespace dtwax

@@ -77,3 +77,49 @@ def test_table_matches_oracle():
         sc, _ = O.sdtw_f16(qh, O.f32_to_f16(lv[k, :47424]))
         want.append(float(O.f16_to_f32(np.array([sc], np.uint16))[0]))
     assert float(f"{want[0]:g}") == fwd and float(f"{want[1]:g}") == rev, (fwd, rev, want)
+
+
+@pytest.mark.gpu
+def test_read_until_fifo_matches_oracle(tmp_path):
+    """The Read Until form (src/ru_main.cu:172-326): 10 x 2048 bytes per batch
+    from the FIFO `pipe` in the working directory, one unsigned sample per
+    byte, mean/std normalised, rows `j j 2048 48128 fwd rev` without a
+    header; the loop ends when the writer closes the FIFO."""
+    import errno
+    import time
+    fifo = tmp_path / "pipe"
+    os.mkfifo(fifo)
+    p = subprocess.Popen([CLI, ARGS[1], ARGS[2], "pipe"], cwd=tmp_path, stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True)
+    data = np.random.default_rng(5).integers(0, 256, 2 * 20480, dtype=np.uint8)
+    deadline = time.time() + 120
+    fd = None
+    while fd is None:  # the driver opens the FIFO once its reference is built
+        try:
+            fd = os.open(fifo, os.O_WRONLY | os.O_NONBLOCK)
+        except OSError as e:
+            assert e.errno == errno.ENXIO and p.poll() is None and time.time() < deadline, \
+                (e, p.poll(), p.stderr.read() if p.poll() is not None else "")
+            time.sleep(0.05)
+    os.set_blocking(fd, True)
+    os.write(fd, data.tobytes())
+    os.close(fd)
+    out, err = p.communicate(timeout=120)
+    assert p.returncode == 0, err
+    rows = [l.split("\t") for l in out.splitlines() if l.strip()]
+    assert len(rows) == 20 and all(len(r) == 6 for r in rows)
+    _, means = load_model_means()
+    fa = fasta_python()
+    rc = fa[::-1].translate(str.maketrans("ACGT", "TGCA"))
+    lv, _ = O.squiggle_reference(fa, rc, means, 48128)
+    sh = [O.f32_to_f16(lv[k, :47424]) for k in range(2)]
+    for i, r in enumerate(rows):
+        b, j = divmod(i, 10)
+        assert r[:4] == [str(j), str(j), "2048", "48128"]
+        sig = data[b * 20480 + j * 2048:b * 20480 + (j + 1) * 2048].astype(np.float32)
+        q, _, _ = O.norm_meanstd(sig, 1e-4)
+        qh = O.f32_to_f16(q)
+        for k in range(2):
+            sc, _ = O.sdtw_f16(qh, sh[k])
+            want = float(O.f16_to_f32(np.array([sc], np.uint16))[0])
+            assert float(f"{want:g}") == float(r[4 + k]), (i, k, r, want)

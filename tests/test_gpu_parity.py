@@ -165,6 +165,43 @@ def test_distances_mirror_clean(ctx):
     assert np.array_equal(got, want)
 
 
+def test_distances_mirror_quirks(ctx):
+    """dtwax_distances with compat_quirks=1 reproduces the reference kernel
+    bit for bit, seam deviations included: equal to the literal emulator on
+    random queries and on exact reference substrings whose zero-cost path
+    crosses a deviating cell (Q1 at row 1024, columns 832k and 832k + 26;
+    Q2 at rows 962..992 of either band, columns 832k), where the clean
+    recurrence would give 0."""
+    rng = np.random.default_rng(67)
+    REF_LEN, TILE, SEG, W = 48128, 832, 26, 32
+    flat = _half_vals(rng.standard_normal((2, REF_LEN)).astype(np.float32)).reshape(2, REF_LEN)
+    tiled = np.zeros((REF_LEN, 2), np.float16)
+    for l in range(REF_LEN // TILE):
+        for i in range(SEG):
+            for k in range(2):
+                tiled[l * TILE + i * W:l * TILE + i * W + W, k] = flat[k, l * TILE + np.arange(W) * SEG + i]
+    starts = [5 * TILE + 26 - 1024,   # Q1 at 832k + 26
+              3 * TILE - 1024,        # Q1 at 832k
+              10 * TILE - 980,        # Q2, band 0
+              20 * TILE - 1994]       # Q2, band 1
+    qs = np.concatenate([rng.standard_normal((3, 2048)).astype(np.float32),
+                         np.stack([flat[0, c:c + 2048] for c in starts])])
+    n = len(qs)
+    qh2 = np.zeros((n * 2048 + 32, 2), np.float16)
+    qh2[: n * 2048, 0] = qs.ravel()
+    qh2[: n * 2048, 1] = qs.ravel()
+    d_ref = D.DeviceArray.from_host(tiled.view(np.uint32).ravel())
+    d_q = D.DeviceArray.from_host(qh2.view(np.uint32).ravel())
+    d_out = D.DeviceArray((n,), np.uint32)
+    ctx.distances(d_ref, d_q, d_out, n, compat_quirks=1)
+    D.lib().dtwax_device_sync()
+    got = d_out.to_host().view(np.float16).reshape(n, 2).astype(np.float32)
+    want = O.emulate_reference(flat, qs)
+    clean = O.emulate_reference(flat, qs, fix_q1=True, fix_q2=True)
+    assert np.all(clean[3:, 0] == 0) and np.all(want[3:, 0] > 0)  # the deviations matter here
+    assert np.array_equal(got, want), (got, want)
+
+
 def test_c2_subset_decisions(ctx):
     """64 reads of the 48.5 kb synthetic workload (3000 raw -> trim 1000 ->
     2000, median/MAD on the GPU): binary16 scores and argmins bit-exact with
