@@ -22,9 +22,10 @@
 // the Read Until state).
 //
 // Reference columns and the top-boundary row are streamed HBM -> LDS ring
-// (256 slots + a 64-slot mirror so a lane reads 64 consecutive slots without
-// wrapping) two blocks ahead with coalesced loads; each step reads its column
-// from LDS with an immediate offset.  No MFMA: this is a min-plus recurrence.
+// (256 slots + an 80-slot mirror so a lane reads the 64 consecutive slots of
+// a block, and the next step's slots one step ahead, without wrapping) two
+// blocks ahead with coalesced loads; each step reads its column from LDS with
+// an immediate offset.  No MFMA: this is a min-plus recurrence.
 //
 // Compiled with -ffp-contract=off (the binary16 form must round t*t and the
 // sum separately, as the reference does) and -fno-honor-nans (no NaN ever
@@ -95,7 +96,8 @@ __device__ __forceinline__ void publish_min(unsigned long long* addr, uint64_t k
 }
 
 constexpr int RING = 256;        // ring slots
-constexpr int RING_ALLOC = 320;  // + mirror of slots [0, 64)
+constexpr int MIRROR = 80;       // slots [0, MIRROR) are also kept at RING + slot
+constexpr int RING_ALLOC = RING + MIRROR;
 
 // ------------------------------------------------------------------------
 // binary16 forms (DTWAX_F16, DTWAX_F16_FUSED, DTWAX_F16_RENORM)
@@ -142,6 +144,41 @@ __device__ __forceinline__ void f16_cells(const F16Band<S, MODE>& B, h2 s, h2 ti
   }
 }
 
+// Per-op binary16 forms (DTWAX_F16, DTWAX_F16_RENORM), software-pipelined:
+// every dependent pair of the chain min -> add -> min -> ... is separated by
+// one independent instruction, in source order pinned by scheduling
+// barriers, so no wait state is needed between them:
+//   cell i:  m = min(tp, dd);  c[i+1] = t[i+1]^2;  v = c[i] + m;  t[i+PG] = fma(...)
+// The fma slot of the last PG cells computes rows 0..PG-1 of the NEXT
+// column (s_next) into tc, which the next step starts from.
+constexpr int PG = 2;
+
+template <int S, int MODE>
+__device__ __forceinline__ void f16_cells_pipe(const F16Band<S, MODE>& B, h2 s, h2 s_next, h2 (&tc)[PG],
+                                               h2 tin, h2 dg, const h2 (&Do)[S], h2 (&Dn)[S]) {
+  h2 t[S], c[S];
+#pragma unroll
+  for (int i = 0; i < PG; i++) t[i] = tc[i];
+  c[0] = t[0] * t[0];
+  __builtin_amdgcn_sched_barrier(0);
+  h2 tp = tin, dd = dg;
+#pragma unroll
+  for (int i = 0; i < S; i++) {
+    const h2 m = hmin2(tp, dd);
+    __builtin_amdgcn_sched_barrier(0);
+    if (i + 1 < S) c[i + 1] = t[i + 1] * t[i + 1];
+    __builtin_amdgcn_sched_barrier(0);
+    const h2 v = c[i] + m;
+    __builtin_amdgcn_sched_barrier(0);
+    if (i + PG < S) t[i + PG] = hfma2(s, B.A[i + PG], B.Qn[i + PG]);
+    else tc[i + PG - S] = hfma2(s_next, B.A[i + PG - S], B.Qn[i + PG - S]);
+    __builtin_amdgcn_sched_barrier(0);
+    Dn[i] = v;
+    dd = Do[i];
+    tp = v;
+  }
+}
+
 struct SweepIO {
   const h2* ref;   // column 0 of the padded reference
   const h2* trow;  // top-boundary row (absolute columns) or null for a free start
@@ -167,18 +204,20 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
   const h2* __restrict__ ref = io.ref;
   const h2* __restrict__ trow = io.trow;
 
-  // Prologue: columns [x-64, x+128) into slots [0, 192) (+ mirror of [0, 64)).
+  // Prologue: columns [x-64, x+128) into slots [0, 192) (+ mirror of [0, MIRROR)).
   {
     h2 a = ref[x - 64 + lane], b = ref[x + lane], c = ref[x + 64 + lane];
     ring_s[lane] = a;
     ring_s[RING + lane] = a;
     ring_s[64 + lane] = b;
+    if (64 + lane < MIRROR) ring_s[RING + 64 + lane] = b;
     ring_s[128 + lane] = c;
     h2 ta = ZERO, tb = ZERO, tc = ZERO;
     if (trow) { ta = trow[x - 64 + lane]; tb = trow[x + lane]; tc = trow[x + 64 + lane]; }
     ring_t[lane] = ta;
     ring_t[RING + lane] = ta;
     ring_t[64 + lane] = tb;
+    if (64 + lane < MIRROR) ring_t[RING + 64 + lane] = tb;
     ring_t[128 + lane] = tc;
   }
   __builtin_amdgcn_s_waitcnt(0);
@@ -192,6 +231,14 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
   if (lane == 0 && io.top_diag) dg = ring_t[63] - K;
   if (INJ && lane == io.inj_lane) tin = ring_t[64 - lane] - K;
 
+  constexpr bool PIPE = MODE != DTWAX_F16_FUSED;
+  // pipelined forms: column of step 0 and its first PG row costs
+  h2 s_cur = ring_s[(64 - lane) & (RING - 1)];
+  h2 tf_cur = ring_t[((64 - lane) & (RING - 1)) + 1];  // top input of step 1 (read a step ahead)
+  h2 tc[PG];
+#pragma unroll
+  for (int i = 0; i < PG; i++) tc[i] = hfma2(s_cur, B.A[i], B.Qn[i]);
+
   for (int32_t T = 0; T < nsteps; T += 64) {
     // Stream block T+128 .. T+191 (written to the ring after this block).
     h2 pf_s = ref[x + T + 128 + lane];
@@ -204,9 +251,21 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
 #pragma unroll
       for (int u = 0; u < 8; u += 2) {
         {
-          h2 s = rs[u];
-          f16_cells<S, MODE>(B, s, tin, dg, D0, D1);
-          h2 tf = rt[u + 1];
+          h2 tf;
+          if constexpr (PIPE) {
+            // next step's column and the top input after it, read a step
+            // ahead (slot base + 65 at most: inside the mirrored ring)
+            const h2 s_next = rs[u + 1];
+            const h2 tf_next = rt[u + 2];
+            f16_cells_pipe<S, MODE>(B, s_cur, s_next, tc, tin, dg, D0, D1);
+            s_cur = s_next;
+            tf = tf_cur;
+            tf_cur = tf_next;
+          } else {
+            h2 s = rs[u];
+            f16_cells<S, MODE>(B, s, tin, dg, D0, D1);
+            tf = rt[u + 1];
+          }
           dg = tin;
           tin = dpp_shr1(tf, D1[S - 1]);
           if (INJ) tin = (lane == io.inj_lane) ? tf : tin;
@@ -214,9 +273,19 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           hist = dpp_shl1(D1[S - 1], hist);
         }
         {
-          h2 s = rs[u + 1];
-          f16_cells<S, MODE>(B, s, tin, dg, D1, D0);
-          h2 tf = rt[u + 2];
+          h2 tf;
+          if constexpr (PIPE) {
+            const h2 s_next = rs[u + 2];
+            const h2 tf_next = rt[u + 3];
+            f16_cells_pipe<S, MODE>(B, s_cur, s_next, tc, tin, dg, D1, D0);
+            s_cur = s_next;
+            tf = tf_cur;
+            tf_cur = tf_next;
+          } else {
+            h2 s = rs[u + 1];
+            f16_cells<S, MODE>(B, s, tin, dg, D1, D0);
+            tf = rt[u + 2];
+          }
           dg = tin;
           tin = dpp_shr1(tf, D0[S - 1]);
           if (INJ) tin = (lane == io.inj_lane) ? tf : tin;
@@ -243,7 +312,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
     const int slot = (T + 192 + lane) & (RING - 1);
     ring_s[slot] = pf_s;
     ring_t[slot] = pf_t;
-    if (slot < 64) { ring_s[RING + slot] = pf_s; ring_t[RING + slot] = pf_t; }
+    if (slot < MIRROR) { ring_s[RING + slot] = pf_s; ring_t[RING + slot] = pf_t; }
     __builtin_amdgcn_wave_barrier();
   }
 }
