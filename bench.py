@@ -39,6 +39,9 @@ def parse():
                     help="c2: 48.5 kb x 4096 x 2000 FP16 (the headline); c4: 16,384 reads of "
                          "U[500, 8000] samples vs a 5 Mb reference, autotuned segment/chunk")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (c4)")
+    ap.add_argument("--ref-chunk", type=int, default=0, help="c4 with --no-tune: reference chunk")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="c4: use --segment (0 = per-read widths) and --ref-chunk instead of the autotuner")
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--precision", default="f16")
     ap.add_argument("--segment", type=int, default=0)
@@ -180,8 +183,11 @@ def main_c4(a):
                                0.0, d_norm.ptr, None, stream))
     D._check(L.dtwax_stream_sync(stream))
     t1 = time.time()
-    best, table = ctx.autotune(d_norm, d_off, lens, sample_reads=512, sample_cols=1 << 20,
-                               cache_path=a.tune_cache, precision=a.precision)
+    if a.no_tune:
+        best, table = dict(segment=a.segment, ref_chunk=a.ref_chunk), []
+    else:
+        best, table = ctx.autotune(d_norm, d_off, lens, sample_reads=512, sample_cols=1 << 20,
+                                   cache_path=a.tune_cache, precision=a.precision)
     tune_s = time.time() - t1
     print(f"[bench] c4 autotune {best} in {tune_s:.1f} s", file=sys.stderr, flush=True)
     plan = ctx.plan(lens, precision=a.precision, **best)

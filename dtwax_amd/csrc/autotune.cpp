@@ -8,11 +8,13 @@
 // chunk (work items per read; each chunk after the first recomputes Q-1
 // overlap columns, while too few items starve the persistent grid) depend on
 // the length distribution and the reference size, so they are measured:
-// every candidate plan runs over a sample of reads (spread evenly over the
-// sorted lengths) against the first sample_cols reference columns, the
-// fastest per counted cell wins, and the choice is cached per device,
-// precision, strand count, reference-size and length-distribution key
-// (in memory, and in a text file when a cache path is given).
+// candidate plans run over a sample of reads (spread evenly over the sorted
+// lengths, replicated until each candidate has 3 items per resident wave)
+// against the first sample_cols reference columns -- widths first, then
+// chunks at the best width -- the fastest per counted cell wins, and the
+// choice is cached per device, precision, strand count, reference-size and
+// length-distribution key (in memory, and in a text file when a cache path
+// is given).
 
 #include <hip/hip_runtime.h>
 
@@ -170,56 +172,115 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
     return set_error(DTWAX_EDEVICE, "autotune upload");
   }
 
+  // 0: per-read widths (the plan's own choice)
   std::vector<int32_t> segs = (o.precision == DTWAX_F16_RENORM) ? std::vector<int32_t>{8, 16}
-                                                                 : std::vector<int32_t>{8, 16, 25, 32};
+                                                                 : std::vector<int32_t>{0, 8, 16, 25, 32};
   if (o.segment) segs = {o.segment};
-  std::vector<int32_t> chunks = {1024, 4096, 16384, 65536, 262144};
-  if (o.ref_chunk) chunks = {o.ref_chunk};
-  double best_rate = -1.0;
-  int32_t best_S = segs[0], best_chunk = chunks[0];
+  // Replicated sample: the full workload has far more work items than the
+  // sample, so a candidate is timed on enough copies of the sample reads to
+  // give it at least 3 items per resident wave (steady state, not the tail).
+  constexpr int32_t kMaxReps = 32;
+  std::vector<int64_t> r_off((size_t)m * kMaxReps);
+  std::vector<int32_t> r_len((size_t)m * kMaxReps);
+  for (int32_t k = 0; k < kMaxReps; k++)
+    for (int32_t j = 0; j < m; j++) {
+      r_off[(size_t)k * m + j] = s_off[(size_t)j];
+      r_len[(size_t)k * m + j] = s_len[(size_t)j];
+    }
+  int64_t* d_roff = nullptr;
+  int32_t* d_rlen = nullptr;
+  dtwax_result* d_rres = nullptr;
+  if (hipMalloc(&d_roff, r_off.size() * 8) != hipSuccess || hipMalloc(&d_rlen, r_len.size() * 4) != hipSuccess ||
+      hipMalloc(&d_rres, r_len.size() * sizeof(dtwax_result)) != hipSuccess ||
+      hipMemcpy(d_roff, r_off.data(), r_off.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_rlen, r_len.data(), r_len.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    for (void* q : {(void*)d_roff, (void*)d_rlen, (void*)d_rres})
+      if (q) (void)hipFree(q);
+    cleanup();
+    return set_error(DTWAX_ENOMEM, "autotune sample buffers");
+  }
+  auto free_rep = [&]() {
+    for (void* q : {(void*)d_roff, (void*)d_rlen, (void*)d_rres})
+      if (q) (void)hipFree(q);
+  };
+
   int32_t nt = 0;
+  // times one candidate; returns counted cells per second (< 0: error)
+  auto time_candidate = [&](int32_t S, int32_t ch, double& rate) -> int {
+    dtwax_score_opts t = o;
+    t.segment = S;
+    t.ref_chunk = ch;
+    dtwax_plan* p = nullptr;
+    int rc = plan_build(c, r_len.data(), m, &t, cols, &p);
+    if (rc) return rc;
+    int64_t grid_sum = 0;
+    for (const auto& g : p->groups) grid_sum += g.grid;
+    if (p->n_items > 0 && p->n_items < 3 * grid_sum) {
+      const int32_t reps = (int32_t)std::min<int64_t>(kMaxReps, (3 * grid_sum + p->n_items - 1) / p->n_items);
+      if (reps > 1) {
+        dtwax_plan_destroy(p);
+        p = nullptr;
+        rc = plan_build(c, r_len.data(), m * reps, &t, cols, &p);
+        if (rc) return rc;
+      }
+    }
+    float ms = 0.f;
+    rc = dtwax_score_plan(p, d_queries, d_roff, d_rlen, d_rres, st);  // warm-up
+    if (!rc) {
+      hipError_t e = hipEventRecord(e0, st);
+      if (e == hipSuccess) rc = dtwax_score_plan(p, d_queries, d_roff, d_rlen, d_rres, st);
+      if (!rc && e == hipSuccess) e = hipEventRecord(e1, st);
+      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (!rc && e != hipSuccess) rc = set_error(DTWAX_EDEVICE, std::string("autotune: ") + hipGetErrorString(e));
+    }
+    rate = ms > 0 ? (double)p->cells / (ms * 1e-3) : 0.0;
+    dtwax_plan_destroy(p);
+    if (rc) return rc;
+    if (nt < table_cap) {
+      table[nt].segment = S;
+      table[nt].ref_chunk = ch;
+      table[nt].ms = ms;
+      table[nt].gcups = (float)(rate / 1e9);
+    }
+    nt++;
+    return DTWAX_OK;
+  };
+
+  // stage 1: widths at one moderate chunk; stage 2: chunks at the best width
+  const int32_t chunk0 = o.ref_chunk ? o.ref_chunk : (int32_t)std::min<int64_t>(65536, cols);
+  double best_rate = -1.0;
+  int32_t best_S = segs[0], best_chunk = chunk0;
   for (int32_t S : segs) {
-    for (int32_t ch : chunks) {
-      if (ch > 2 * cols && ch != chunks.front()) continue;  // longer than the sample: same as shorter ones
-      dtwax_score_opts t = o;
-      t.segment = S;
-      t.ref_chunk = ch;
-      dtwax_plan* p = nullptr;
-      int rc = plan_build(c, s_len.data(), m, &t, cols, &p);
+    double rate = 0;
+    int rc = time_candidate(S, chunk0, rate);
+    if (rc) {
+      free_rep();
+      cleanup();
+      return rc;
+    }
+    if (rate > best_rate) {
+      best_rate = rate;
+      best_S = S;
+    }
+  }
+  if (!o.ref_chunk) {
+    for (int32_t ch : {1024, 4096, 16384, 131072, 262144}) {
+      if (ch > 2 * cols) continue;  // longer than the sample: same as shorter ones
+      double rate = 0;
+      int rc = time_candidate(best_S, ch, rate);
       if (rc) {
+        free_rep();
         cleanup();
         return rc;
       }
-      float ms = 0.f;
-      rc = dtwax_score_plan(p, d_queries, d_soff, d_slen, d_res, st);  // warm-up
-      if (!rc) {
-        hipError_t e = hipEventRecord(e0, st);
-        if (e == hipSuccess) rc = dtwax_score_plan(p, d_queries, d_soff, d_slen, d_res, st);
-        if (!rc && e == hipSuccess) e = hipEventRecord(e1, st);
-        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
-        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (!rc && e != hipSuccess) rc = set_error(DTWAX_EDEVICE, std::string("autotune: ") + hipGetErrorString(e));
-      }
-      const double rate = ms > 0 ? (double)p->cells / (ms * 1e-3) : 0.0;
-      dtwax_plan_destroy(p);
-      if (rc) {
-        cleanup();
-        return rc;
-      }
-      if (nt < table_cap) {
-        table[nt].segment = S;
-        table[nt].ref_chunk = ch;
-        table[nt].ms = ms;
-        table[nt].gcups = (float)(rate / 1e9);
-      }
-      nt++;
       if (rate > best_rate) {
         best_rate = rate;
-        best_S = S;
         best_chunk = ch;
       }
     }
   }
+  free_rep();
   cleanup();
   if (n_table) *n_table = std::min(nt, table_cap);
   {

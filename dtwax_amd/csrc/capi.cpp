@@ -255,22 +255,58 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   p->n_reads = n_reads;
   p->precision = o.precision;
   p->nstr = c->n_strands;
-  // the renormalised form subtracts each band's exact row minimum, so it
-  // defaults to the shortest bands (64 x 8 rows)
-  p->S = o.segment ? o.segment : (o.precision == DTWAX_F16_RENORM ? 8 : auto_segment(lens));
   // initial per-row drift; each band then adds its exact row minimum / 64 per
   // lane.  Measured against FP64 over Q = 1000..8000 (both classes): 0.01
   // keeps the max relative error below 1%; 0.1 over-subtracts for reads
   // whose row minimum grows slowly (targets) and loses up to 15% at Q = 1000.
   const float drift = o.renorm_drift > 0 ? o.renorm_drift : 0.01f;
-  p->renorm_k = drift * (float)p->S;
   const int64_t N = col_limit;
-  const int64_t BR = 64 * p->S;
-  bool multiband = false;
-  for (int32_t q : lens) multiband |= q > BR;
 
-  const int occ = std::max(1, dtwax::occupancy(p->precision, p->S, p->nstr, false));
-  const int64_t slots = (int64_t)occ * c->n_cu;
+  // Segment width per read.  Fixed when given; the renormalised form
+  // subtracts each band's exact row minimum, so it defaults to the shortest
+  // bands (64 x 8 rows); binary32 takes one width for the batch; the
+  // binary16 forms pick, per read, the width with the least padded work:
+  // padded rows x the measured time per computed cell of that width
+  // (relative to S = 32: 1.072 at 8, 1.021 at 16, 1.026 at 25) x 1.5% per
+  // extra band (band hand-over through scratch).
+  std::vector<int> seg_of((size_t)n_reads, 0);
+  if (o.segment) {
+    std::fill(seg_of.begin(), seg_of.end(), o.segment);
+  } else if (o.precision == DTWAX_F16_RENORM) {
+    std::fill(seg_of.begin(), seg_of.end(), 8);
+  } else if (o.precision == DTWAX_F32) {
+    std::fill(seg_of.begin(), seg_of.end(), auto_segment(lens));
+  } else {
+    static const int cand[4] = {8, 16, 25, 32};
+    static const double f0[4] = {1.072, 1.021, 1.026, 1.0};
+    auto cost = [&](int64_t q, int ci) {
+      const int64_t BR = 64 * cand[ci];
+      const int64_t nb = (q + BR - 1) / BR;
+      return (double)(nb * BR) * f0[ci] * (1.0 + 0.015 * (double)(nb - 1));
+    };
+    for (int32_t r = 0; r < n_reads; r++) {
+      int best = 0;
+      for (int ci = 1; ci < 4; ci++)
+        if (cost(lens[r], ci) < cost(lens[r], best) - 1e-9) best = ci;
+      seg_of[(size_t)r] = cand[best];
+    }
+  }
+  std::vector<int> widths;
+  for (int32_t r = 0; r < n_reads; r++)
+    if (std::find(widths.begin(), widths.end(), seg_of[(size_t)r]) == widths.end()) widths.push_back(seg_of[(size_t)r]);
+  std::sort(widths.rbegin(), widths.rend());
+  if (widths.empty()) widths.push_back(o.segment ? o.segment : 32);
+  p->S = widths.size() == 1 ? widths[0] : 0;
+  p->renorm_k = drift * (float)widths[0];
+
+  bool multiband = false;
+  int64_t slots = 1, slots_sum = 0;
+  for (int32_t r = 0; r < n_reads; r++) multiband |= lens[r] > 64 * seg_of[(size_t)r];
+  for (int S : widths) {
+    const int64_t sl = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, false)) * c->n_cu;
+    slots = std::max(slots, sl);
+    slots_sum += sl;
+  }
 
   // chunking: explicit, or whole strands unless the GPU would be underfilled
   // or the band scratch would not fit the budget.
@@ -284,60 +320,86 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     if (multiband) {
       const int64_t budget = (int64_t)8 << 30;  // bytes of band scratch
       const int64_t bytes_per_col = 4 * (p->precision == DTWAX_F32 ? p->nstr : 1);
-      const int64_t max_w = budget / (std::min<int64_t>(slots, (int64_t)n_reads * ((N + chunk - 1) / chunk)) * bytes_per_col);
+      const int64_t max_w = budget / (std::min<int64_t>(slots_sum, (int64_t)n_reads * ((N + chunk - 1) / chunk)) * bytes_per_col);
       if (chunk + qmax > max_w) chunk = std::max<int64_t>(4096, max_w - qmax);
     }
   }
   chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, N));
   p->chunk = (int32_t)chunk;
 
-  std::vector<std::pair<int64_t, WorkItem>> items;
-  int64_t max_w = 0;
-  const int strands_counted = c->n_strands;
-  for (int32_t r = 0; r < n_reads; r++) {
-    const int64_t q = lens[r];
-    const int64_t nb = (q + BR - 1) / BR;
-    for (int64_t c0 = 0; c0 < N; c0 += chunk) {
-      const int64_t c1 = std::min<int64_t>(N, c0 + chunk);
-      const int64_t x = std::max<int64_t>(0, c0 - (q - 1));
-      WorkItem it{};
-      it.read = r;
-      it.slot = 0;
-      it.x = (int32_t)x;
-      it.c0 = (int32_t)c0;
-      it.c1 = (int32_t)c1;
-      const int64_t steps = ((c1 - x + 63) + 63) / 64 * 64;
-      const int64_t cost = nb * steps;
-      items.push_back({cost, it});
-      p->cells += q * (c1 - c0) * strands_counted;
-      p->computed += nb * BR * steps * (p->precision == DTWAX_F32 ? p->nstr : 2);
-      max_w = std::max(max_w, c1 - x);
-    }
-  }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const std::pair<int64_t, WorkItem>& a, const std::pair<int64_t, WorkItem>& b) {
-                     return a.first > b.first;
-                   });
-  p->n_items = (int32_t)items.size();
-  p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(slots, (int64_t)items.size()));
-
   std::vector<WorkItem> hv;
-  hv.reserve(items.size());
-  for (auto& pr : items) hv.push_back(pr.second);
+  int64_t max_w = 0, scratch_rows = 0;
+  int max_grid = 1;
+  const int strands_counted = c->n_strands;
+  for (int S : widths) {
+    const int64_t BR = 64 * S;
+    std::vector<std::pair<int64_t, WorkItem>> items;
+    for (int32_t r = 0; r < n_reads; r++) {
+      if (seg_of[(size_t)r] != S) continue;
+      const int64_t q = lens[r];
+      const int64_t nb = (q + BR - 1) / BR;
+      for (int64_t c0 = 0; c0 < N; c0 += chunk) {
+        const int64_t c1 = std::min<int64_t>(N, c0 + chunk);
+        const int64_t x = std::max<int64_t>(0, c0 - (q - 1));
+        WorkItem it{};
+        it.read = r;
+        it.slot = 0;
+        it.x = (int32_t)x;
+        it.c0 = (int32_t)c0;
+        it.c1 = (int32_t)c1;
+        const int64_t steps = ((c1 - x + 63) + 63) / 64 * 64;
+        const int64_t cost = nb * steps;
+        items.push_back({cost, it});
+        p->cells += q * (c1 - c0) * strands_counted;
+        p->computed += nb * BR * steps * (p->precision == DTWAX_F32 ? p->nstr : 2);
+        max_w = std::max(max_w, c1 - x);
+      }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<int64_t, WorkItem>& a, const std::pair<int64_t, WorkItem>& b) {
+                       return a.first > b.first;
+                     });
+    LaunchGroup g;
+    g.S = S;
+    g.item0 = (int32_t)hv.size();
+    g.n_items = (int32_t)items.size();
+    const int64_t slots_S = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, false)) * c->n_cu;
+    g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(slots_S, (int64_t)items.size()));
+    g.renorm_k = drift * (float)S;
+    g.scratch_wg0 = scratch_rows;  // concurrent groups: disjoint band scratch
+    scratch_rows += g.grid;
+    max_grid = std::max(max_grid, g.grid);
+    for (auto& pr : items) hv.push_back(pr.second);
+    p->groups.push_back(g);
+  }
+  p->n_items = (int32_t)hv.size();
+  p->grid = max_grid;
+
   auto cleanup = [&](const std::string& m) {
     dtwax_plan_destroy(p);
     return fail(DTWAX_ENOMEM, m);
   };
   if (hipMalloc(&p->d_items, std::max<size_t>(1, hv.size()) * sizeof(WorkItem)) != hipSuccess)
     return cleanup("item allocation failed");
-  if (hipMalloc(&p->d_counter, sizeof(int32_t)) != hipSuccess) return cleanup("counter allocation failed");
+  if (hipMalloc(&p->d_counter, p->groups.size() * sizeof(int32_t) + sizeof(int32_t)) != hipSuccess)
+    return cleanup("counter allocation failed");
   if (hipMalloc(&p->d_keys, std::max<size_t>(1, (size_t)n_reads * 2) * sizeof(unsigned long long)) != hipSuccess)
     return cleanup("key allocation failed");
   if (!hv.empty()) HIP_TRY(hipMemcpy(p->d_items, hv.data(), hv.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  if (p->groups.size() > 1) {
+    if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess)
+      return cleanup("event creation failed");
+    for (size_t g = 1; g < p->groups.size(); g++)
+      // non-blocking: no implicit ordering with a NULL caller stream, only
+      // the fork/join events
+      if (hipStreamCreateWithFlags(&p->groups[g].stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&p->groups[g].done, hipEventDisableTiming) != hipSuccess)
+        return cleanup("stream creation failed");
+  }
   if (multiband) {
     const int64_t epc = (p->precision == DTWAX_F32) ? p->nstr : 1;  // 4-byte elements per column
     p->scratch_stride = COL_PADL + max_w + COL_PADR;
-    const size_t bytes = (size_t)p->grid * (size_t)p->scratch_stride * (size_t)epc * 4;
+    const size_t bytes = (size_t)scratch_rows * (size_t)p->scratch_stride * (size_t)epc * 4;
     if (hipMalloc(&p->d_scratch, bytes) != hipSuccess) return cleanup("band scratch allocation failed");
     HIP_TRY(hipMemset(p->d_scratch, 0, bytes));
   }
@@ -353,6 +415,11 @@ void dtwax_plan_destroy(dtwax_plan* p) {
   (void)hipDeviceSynchronize();
   for (void* q : {(void*)p->d_items, (void*)p->d_counter, (void*)p->d_keys, p->d_scratch})
     if (q) (void)hipFree(q);
+  for (auto& g : p->groups) {
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    if (g.done) (void)hipEventDestroy(g.done);
+  }
+  if (p->fork) (void)hipEventDestroy(p->fork);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   delete p;
@@ -394,7 +461,15 @@ int dtwax_plan_kernel_ms(dtwax_plan* p, float* ms) {
 namespace dtwax {
 int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
   HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(launch_reset(L.counter, L.keys, (int64_t)L.n_reads * 2, st));
+  LaunchGroup single;
+  single.S = L.S;
+  single.item0 = 0;
+  single.n_items = L.n_items;
+  single.grid = L.grid;
+  single.renorm_k = L.renorm_k;
+  const LaunchGroup* groups = L.n_groups ? L.groups : &single;
+  const int n_groups = L.n_groups ? L.n_groups : 1;
+  HIP_TRY(launch_reset(L.counter, n_groups, L.keys, (int64_t)L.n_reads * 2, st));
   KParams k{};
   k.ref = (L.precision == DTWAX_F32)
               ? (const void*)((const float*)c->ref32_alloc + (int64_t)COL_PADL * c->n_strands)
@@ -411,12 +486,31 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
   k.scratch_stride = L.scratch_stride;
   k.state = L.state;
   k.row_stride = L.row_stride;
-  k.renorm_k = L.renorm_k;
-  if (k.n_items > 0) {
-    if (L.ev0) HIP_TRY(hipEventRecord(L.ev0, st));
-    HIP_TRY(launch_sdtw(k, L.precision, L.S, L.nstr, std::max(1, std::min(L.grid, L.n_items)), false, st));
-    if (L.ev1) HIP_TRY(hipEventRecord(L.ev1, st));
+  if (L.ev0) HIP_TRY(hipEventRecord(L.ev0, st));
+  // Groups of different segment widths run concurrently: group 0 on the
+  // caller's stream, the others on their own streams forked from it and
+  // joined back, so a persistent launch that drains early leaves its compute
+  // units to the next width instead of idling them.
+  if (n_groups > 1) HIP_TRY(hipEventRecord(L.fork, st));
+  const int64_t epc = (L.precision == DTWAX_F32) ? L.nstr : 1;
+  for (int g = 0; g < n_groups; g++) {
+    const LaunchGroup& G = groups[g];
+    hipStream_t gs = (g == 0 || !G.stream) ? st : G.stream;
+    if (gs != st) HIP_TRY(hipStreamWaitEvent(gs, L.fork, 0));
+    if (G.n_items > 0) {
+      k.items = L.items + G.item0;
+      k.n_items = G.n_items;
+      k.counter = L.counter + g;
+      k.renorm_k = G.renorm_k;
+      k.scratch = L.scratch ? (void*)((float*)L.scratch + G.scratch_wg0 * L.scratch_stride * epc) : nullptr;
+      HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, std::max(1, std::min(G.grid, G.n_items)), false, gs));
+    }
+    if (gs != st) {
+      HIP_TRY(hipEventRecord(G.done, gs));
+      HIP_TRY(hipStreamWaitEvent(st, G.done, 0));
+    }
   }
+  if (L.ev1) HIP_TRY(hipEventRecord(L.ev1, st));
   HIP_TRY(launch_finalize(L.keys, L.n_reads, L.precision == DTWAX_F32 ? L.nstr : 2, L.out, st));
   return DTWAX_OK;
 }
@@ -442,6 +536,9 @@ static int run_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const
   L.scratch = p->d_scratch;
   L.scratch_stride = p->scratch_stride;
   L.renorm_k = p->renorm_k;
+  L.groups = p->groups.data();
+  L.n_groups = (int)p->groups.size();
+  L.fork = p->fork;
   L.out = reinterpret_cast<DeviceResult*>(d_out);
   if (p->timing) {
     L.ev0 = p->ev0;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "dtwax.h"
 #include "kernels.h"
@@ -19,9 +20,22 @@ int set_error(int code, const std::string& msg);
       return ::dtwax::set_error(DTWAX_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// One scoring launch: reset keys and counter, persistent sDTW kernel over
-// `n_items` items, finalise `n_reads` results.  ev0/ev1 (optional) bracket
-// the sDTW kernel on `st`.
+// A run of work items sharing one segment width (one persistent launch).
+struct LaunchGroup {
+  int S = 32;
+  int32_t item0 = 0;   // first item in the item array
+  int32_t n_items = 0;
+  int grid = 1;
+  float renorm_k = 0.f;
+  int64_t scratch_wg0 = 0;  // first band-scratch row (workgroup slot) of this group
+  hipStream_t stream = nullptr;  // groups after the first run on their own stream
+  hipEvent_t done = nullptr;     // recorded on that stream after the launch
+};
+
+// One scoring pass: reset keys and counters, one persistent sDTW launch per
+// group (in order, on `st`), finalise `n_reads` results.  Without groups the
+// fields S/items/n_items/grid/renorm_k describe the single launch.  ev0/ev1
+// (optional) bracket the sDTW launches.
 struct ScoreLaunch {
   int precision = DTWAX_F16;
   int S = 32;
@@ -42,6 +56,9 @@ struct ScoreLaunch {
   float renorm_k = 0.f;
   DeviceResult* out = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const LaunchGroup* groups = nullptr;  // counter[g] for group g
+  int n_groups = 0;
+  hipEvent_t fork = nullptr;  // recorded on the caller's stream before the groups start
 };
 
 }  // namespace dtwax
@@ -72,7 +89,9 @@ struct dtwax_plan {
   dtwax_ctx* ctx = nullptr;
   int32_t n_reads = 0;
   int precision = DTWAX_F16;
-  int S = 32;
+  int S = 32;  // 0 when the reads are split over several segment widths
+  std::vector<dtwax::LaunchGroup> groups;  // launched concurrently, one stream each
+  hipEvent_t fork = nullptr;
   int nstr = 2;
   int32_t chunk = 0;
   bool inj = false;

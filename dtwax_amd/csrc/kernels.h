@@ -60,7 +60,9 @@ struct KParams {
 
 hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool inj,
                        hipStream_t st);
-hipError_t launch_reset(int32_t* counter, unsigned long long* keys, int64_t n_keys, hipStream_t st);
+// zero n_counters work counters (n_counters <= 256) and set n_keys keys to ~0
+hipError_t launch_reset(int32_t* counters, int32_t n_counters, unsigned long long* keys, int64_t n_keys,
+                        hipStream_t st);
 hipError_t launch_finalize(const unsigned long long* keys, int32_t n_reads, int32_t n_strands,
                            DeviceResult* out, hipStream_t st);
 hipError_t launch_selftest_lanes(int32_t* d_out, hipStream_t st);

@@ -603,9 +603,10 @@ __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
 // ------------------------------------------------------------------------
 // small helpers
 // ------------------------------------------------------------------------
-__global__ void reset_score_kernel(int32_t* counter, unsigned long long* keys, int64_t n_keys) {
+__global__ void reset_score_kernel(int32_t* counters, int32_t n_counters, unsigned long long* keys,
+                                   int64_t n_keys) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *counter = 0;
+  if (i < n_counters) counters[i] = 0;
   for (; i < n_keys; i += (int64_t)gridDim.x * blockDim.x) keys[i] = ~0ull;
 }
 
@@ -693,11 +694,13 @@ hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int gri
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_reset(int32_t* counter, unsigned long long* keys, int64_t n_keys, hipStream_t st) {
+hipError_t launch_reset(int32_t* counters, int32_t n_counters, unsigned long long* keys, int64_t n_keys,
+                        hipStream_t st) {
   int64_t blocks = (n_keys + 255) / 256;
   if (blocks < 1) blocks = 1;
   if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(reset_score_kernel, dim3((unsigned)blocks), dim3(256), 0, st, counter, keys, n_keys);
+  hipLaunchKernelGGL(reset_score_kernel, dim3((unsigned)blocks), dim3(256), 0, st, counters, n_counters, keys,
+                     n_keys);
   return hipGetLastError();
 }
 

@@ -69,7 +69,10 @@ typedef struct {
 
 typedef struct {
   int32_t precision;     /* dtwax_precision */
-  int32_t segment;       /* query rows per lane (8, 16, 25 or 32); 0 = automatic */
+  int32_t segment;       /* query rows per lane (8, 16, 25 or 32) for every read; 0 = automatic:
+                            per read for DTWAX_F16 and DTWAX_F16_FUSED (the width with the
+                            least padded work, one persistent launch per width used), 8 for
+                            DTWAX_F16_RENORM, one width per batch for DTWAX_F32 */
   int32_t ref_chunk;     /* reference columns per work item; 0 = automatic (whole strand when
                             it fits); chunks overlap by (query length - 1) columns, exact for
                             the no-deletion recurrence */
@@ -153,25 +156,30 @@ int dtwax_plan_create(dtwax_ctx* ctx, const int32_t* h_lens, int32_t n_reads,
 void dtwax_plan_destroy(dtwax_plan* plan);
 /* cells = sum over reads of len * counted columns * strands (the GCUPS
  * numerator); computed = cells actually evaluated (padding rows, chunk
- * overlap and the 63-step wavefront ramp included). */
+ * overlap and the 63-step wavefront ramp included); segment = the width of
+ * every read, or 0 if the plan mixes widths; grid = the largest launch. */
 int dtwax_plan_info(const dtwax_plan* plan, int64_t* cells, int64_t* computed,
                     int32_t* n_items, int32_t* segment, int32_t* ref_chunk, int32_t* grid);
 int dtwax_score_plan(dtwax_plan* plan, const float* d_queries, const int64_t* d_offsets,
                      const int32_t* d_lens, dtwax_result* d_out, void* stream);
-/* Enable HIP-event timing of the sDTW kernel launch inside dtwax_score_plan;
- * dtwax_plan_kernel_ms then waits for and returns the last launch's time. */
+/* Enable HIP-event timing of the sDTW kernel launches inside
+ * dtwax_score_plan; dtwax_plan_kernel_ms then waits for and returns the last
+ * call's sDTW time (all of its launches). */
 int dtwax_plan_set_timing(dtwax_plan* plan, int32_t enable);
 int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
 
 /* ---- autotuner -----------------------------------------------------------
  * Picks segment and ref_chunk for a workload (no counterpart in the
  * reference, whose geometry is fixed at compile time, src/include/common.hpp:
- * 60-98): times the sDTW kernel for every candidate (segment in {8, 16, 25,
- * 32} ({8, 16} for DTWAX_F16_RENORM) x ref_chunk in {1024, 4096, 16384,
- * 65536, 262144}; a nonzero base->segment or base->ref_chunk fixes that
- * one) on sample_reads reads (<= 0: 512) spread over the sorted lengths,
- * against the first sample_cols reference columns (<= 0: 2^20), and writes
- * the fastest per counted cell into *best (base's other fields kept).
+ * 60-98) by timing the sDTW kernel on a sample: sample_reads reads (<= 0:
+ * 512) spread over the sorted lengths, against the first sample_cols
+ * reference columns (<= 0: 2^20), replicated until every candidate has at
+ * least 3 work items per resident wave (steady state, as in the full run).
+ * Stage 1 times segment in {0 = per-read widths, 8, 16, 25, 32} ({8, 16} for
+ * DTWAX_F16_RENORM) at ref_chunk min(65536, sample_cols); stage 2 times
+ * ref_chunk in {1024, 4096, 16384, 131072, 262144} at the best segment.  A
+ * nonzero base->segment or base->ref_chunk fixes that one.  The fastest per
+ * counted cell goes into *best (base's other fields kept).
  * d_queries/d_offsets: the workload's device queries (as for dtwax_score);
  * h_lens: host lengths.  The choice is cached per device, options, strand
  * count, reference-size and length-distribution key, in memory and, if

@@ -222,6 +222,28 @@ def test_c2_subset_decisions(ctx):
     assert acc >= 0.9, acc
 
 
+def test_mixed_segment_plan_exact(ctx):
+    """Automatic widths split uneven reads over several persistent launches
+    (one per width); scores and argmins stay bit-exact with the oracle."""
+    rng = np.random.default_rng(91)
+    ref = rng.standard_normal((2, 4001)).astype(np.float32)
+    ctx.upload_reference(ref)
+    # lengths whose least-padding widths are 8, 16, 25 and 32; 64-column
+    # pieces give every width group enough work items to keep its own launch
+    lens = [500] * 240 + [1000] * 240 + [1600] * 240 + [2048] * 240 + [777, 3001]
+    reads = [rng.standard_normal(n).astype(np.float32) for n in lens]
+    flat, offs, lns = D.pack_reads(reads)
+    p = ctx.plan(lns, precision="f16", segment=0, ref_chunk=64)
+    assert p.info()["segment"] == 0  # several widths in this plan
+    d_q, d_off, d_len = (D.DeviceArray.from_host(x) for x in (flat, offs, lns))
+    d_out = D.DeviceArray((len(reads),), D.RESULT_DTYPE)
+    p.run(d_q, d_off, d_len, d_out, None)
+    res = d_out.to_host()
+    p.close()
+    osc, oam = _oracle_f16(reads, ref)
+    assert np.array_equal(res["score"], osc) and np.array_equal(res["argmin"], oam)
+
+
 def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     """The autotuner times every candidate on a sample, caches its choice
     (memory and file), and the chosen plan scores exactly like any other."""
@@ -235,7 +257,7 @@ def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     cache = tmp_path / "tune.txt"
     best, table = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, cache_path=cache)
     assert len(table) >= 8
-    assert best["segment"] in (8, 16, 25, 32) and best["ref_chunk"] in (1024, 4096, 16384, 65536, 262144)
+    assert best["segment"] in (0, 8, 16, 25, 32) and best["ref_chunk"] > 0
     top = table[np.argmax(table["gcups"])]
     assert (top["segment"], top["ref_chunk"]) == (best["segment"], best["ref_chunk"])
     assert cache.read_text().split()[-2:] == [str(best["segment"]), str(best["ref_chunk"])]
