@@ -595,6 +595,8 @@ int dtwax_ru_process(dtwax_ru* r, const dtwax_ru_chunk* chunks, int32_t n, const
         return set_error(DTWAX_EINVAL, "two chunks of one channel in one batch");
     }
   }
+  if (r->ctx->n_cols != r->n_cols || r->ctx->n_strands != r->n_strands)
+    return set_error(DTWAX_ENOTREADY, "the context's reference changed after dtwax_ru_create");
   std::lock_guard<std::mutex> bk(r->book_mu);
   std::vector<int32_t> idx, channels;
   std::vector<const float*> src;

@@ -266,3 +266,17 @@ def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     sc, am = ctx.score(reads, precision="f16", **best)
     sc0, am0 = ctx.score(reads, precision="f16", segment=32, ref_chunk=0)
     assert np.array_equal(sc, sc0) and np.array_equal(am, am0)
+
+
+def test_plan_and_score_argument_errors(ctx):
+    """Bad options fail with a status and a message, not a fault."""
+    rng = np.random.default_rng(4)
+    ctx.upload_reference(rng.standard_normal((2, 1000)).astype(np.float32))
+    for kw in (dict(segment=12), dict(ref_chunk=-1), dict(precision=7)):
+        with pytest.raises(D.DtwaxError):
+            ctx.plan(np.array([100, 200], np.int32), **kw)
+    with pytest.raises(D.DtwaxError, match="at least one sample"):
+        ctx.plan(np.array([100, 0], np.int32))
+    q = [rng.standard_normal(50).astype(np.float32)]
+    sc, am = ctx.score(q, precision="f16", segment=8)
+    assert sc.shape == (1, 2) and np.all(np.isfinite(sc))

@@ -267,3 +267,28 @@ def test_ru_realtime_and_plain_forms(ctx):
     assert rt["reads_decided"] > 30 and rt["accuracy"] > 0.85
     assert plain["reads_decided"] > 10
     assert rt["reads_decided_per_s"] > plain["reads_decided_per_s"]
+
+
+def test_ru_admission_by_hbm_budget_and_errors(ctx):
+    """Slots come from the HBM budget (2 rows x (n_cols + 448) x 4 B each);
+    bad options and a reference swapped under a live engine are errors."""
+    rng = np.random.default_rng(3)
+    ref = rng.standard_normal((2, 5000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    slot_bytes = 2 * (5000 + 448) * 4
+    ru = D.ReadUntil(ctx, max_reads=64, chunk_len=1600, hbm_budget=3 * slot_bytes + 17)
+    try:
+        info = ru.info()
+        assert info["slots"] == 3 and info["state_bytes"] == 3 * slot_bytes
+        d = ru.process([0, 1, 2, 3], [1, 1, 1, 1], rng.standard_normal((4, 1600)).astype(np.float32))
+        assert list(d["action"][:3]) == [D.RU_CONTINUE] * 3 and d["action"][3] == D.RU_NO_SLOT
+        ctx.upload_reference(rng.standard_normal((2, 4000)).astype(np.float32))
+        with pytest.raises(D.DtwaxError, match="reference changed"):
+            ru.process([5], [1], np.zeros((1, 1600), np.float32))
+    finally:
+        ru.close()
+    ctx.upload_reference(ref)
+    for kw in (dict(chunk_len=1000), dict(chunk_len=1600, segment=32), dict(precision="f16_renorm"),
+               dict(n_streams=9), dict(hbm_budget=100)):
+        with pytest.raises(D.DtwaxError):
+            D.ReadUntil(ctx, **kw)
