@@ -123,3 +123,35 @@ def test_read_until_fifo_matches_oracle(tmp_path):
             sc, _ = O.sdtw_f16(qh, sh[k])
             want = float(O.f16_to_f32(np.array([sc], np.uint16))[0])
             assert float(f"{want:g}") == float(r[4 + k]), (i, k, r, want)
+
+
+@pytest.mark.gpu
+def test_fp32_build_and_native_modes():
+    """--fp32-build: the reference's FP32 build (one concatenated fwd|rev
+    sequence of 2 x 48,128 levels, 115 tiles of 832 scored, one score column);
+    --native --precision f32 --norm medmad: every k-mer, both strands, binary32."""
+    _, means = load_model_means()
+    fa = fasta_python()
+    rc = fa[::-1].translate(str.maketrans("ACGT", "TGCA"))
+    sig = np.load(os.path.join(GOLDEN, "batch0_signal.npy")).astype(np.float32)[1952:4000]
+    # FP32 build
+    p = run("--fp32-build")
+    lines = p.stdout.splitlines()
+    assert "Read_ID\tQUERY_LEN\tREF_LEN\tsDTW-score" in lines
+    row = [l.split("\t") for l in lines if re.match(r"^\d+\tread_", l)][0]
+    assert row[2] == "2048" and row[3] == "96256" and len(row) == 5
+    lv, _ = O.squiggle_reference(fa, rc, means, 48128)
+    cat = np.concatenate([lv[0], lv[1]])[: (2 * 48128 // 832) * 832]
+    q, _, _ = O.norm_meanstd(sig, 1e-4)
+    want, _ = O.sdtw_f32(q, cat)
+    assert float(f"{want:g}") == float(row[4]), (row, want)
+    # native, binary32, median/MAD
+    p = run("--native", "--precision", "f32", "--norm", "medmad")
+    row = [l.split("\t") for l in p.stdout.splitlines() if re.match(r"^\d+\tread_", l)][0]
+    n_k = len(fa) - 5
+    lvn, _ = O.squiggle_reference(fa, rc, means, n_k)
+    qm, _, _ = O.norm_medmad(sig)
+    assert row[3] == str(n_k)
+    for k in range(2):
+        w, _ = O.sdtw_f32(qm, lvn[k])
+        assert float(f"{w:g}") == float(row[4 + k]), (k, row, w)
