@@ -375,6 +375,25 @@ def main():
         "timing": {"wall_s": round(wall, 3), "gen_s": round(gen_s, 2)},
     }
 
+    # the other arithmetic forms on the same normalised reads (kernel time of
+    # one timed launch after one warm-up; FP16_FUSED rounds t*t + min once,
+    # FP16_RENORM subtracts drift, FP32 is the reference's FP32 build)
+    forms = {}
+    for prec in ("f16_fused", "f16_renorm", "f32"):
+        if prec == a.precision:
+            continue
+        fp = ctx.plan(lens, precision=prec)
+        fp.set_timing(True)
+        fms = []
+        for _ in range(2):
+            fp.run(d_norm, d_off, d_len, d_out, stream)
+            fms.append(fp.kernel_ms())
+        fi = fp.info()
+        forms[prec] = {"kernel_ms": round(fms[-1], 3), "segment": fi["segment"],
+                       "kernel_gcups": round(fi["cells"] / (fms[-1] * 1e-3) / 1e9, 1)}
+        fp.close()
+    out["forms"] = forms
+
     if not a.no_cpu:
         norm = d_norm.to_host()
         qs = [norm[o:o + n] for o, n in zip(offs, lens)]
