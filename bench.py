@@ -377,19 +377,22 @@ def main():
 
     # the other arithmetic forms on the same normalised reads (kernel time of
     # one timed launch after one warm-up; FP16_FUSED rounds t*t + min once,
-    # FP16_RENORM subtracts drift, FP32 is the reference's FP32 build)
+    # FP16_RENORM subtracts drift, FP32 is the reference's FP32 build;
+    # "f16_moves3" is the reference's build without -DNO_REF_DEL, whose cell
+    # also takes the left neighbour, over whole strands)
     forms = {}
-    for prec in ("f16_fused", "f16_renorm", "f32"):
-        if prec == a.precision:
+    for name, prec, moves in (("f16_fused", "f16_fused", 2), ("f16_renorm", "f16_renorm", 2),
+                              ("f32", "f32", 2), ("f16_moves3", "f16", 3)):
+        if prec == a.precision and moves == 2:
             continue
-        fp = ctx.plan(lens, precision=prec)
+        fp = ctx.plan(lens, precision=prec, moves=moves)
         fp.set_timing(True)
         fms = []
         for _ in range(2):
             fp.run(d_norm, d_off, d_len, d_out, stream)
             fms.append(fp.kernel_ms())
         fi = fp.info()
-        forms[prec] = {"kernel_ms": round(fms[-1], 3), "segment": fi["segment"],
+        forms[name] = {"kernel_ms": round(fms[-1], 3), "segment": fi["segment"],
                        "kernel_gcups": round(fi["cells"] / (fms[-1] * 1e-3) / 1e9, 1)}
         fp.close()
     out["forms"] = forms

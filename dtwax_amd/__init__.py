@@ -30,7 +30,7 @@ class DtwaxError(RuntimeError):
 class ScoreOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("segment", C.c_int32), ("ref_chunk", C.c_int32),
                 ("renorm_drift", C.c_float), ("compat_quirks", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("moves", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 RESULT_DTYPE = np.dtype([("score", np.float32, 2), ("argmin", np.int32, 2)])
@@ -168,10 +168,11 @@ def pack_reads(reads):
     return np.ascontiguousarray(flat, np.float32), offs, lens
 
 
-def make_opts(precision=F16, segment=0, ref_chunk=0, renorm_drift=0.0):
+def make_opts(precision=F16, segment=0, ref_chunk=0, renorm_drift=0.0, moves=2):
     if isinstance(precision, str):
         precision = PRECISIONS[precision]
-    return ScoreOpts(int(precision), int(segment), int(ref_chunk), float(renorm_drift), 0)
+    return ScoreOpts(int(precision), int(segment), int(ref_chunk), float(renorm_drift), 0,
+                     int(moves))
 
 
 class Plan:

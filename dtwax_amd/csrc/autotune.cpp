@@ -50,7 +50,7 @@ std::string tune_key(const dtwax_ctx* c, const dtwax_score_opts& o, const std::v
   const size_t n = sorted_lens.size();
   auto q = [&](double f) { return sorted_lens[std::min(n - 1, (size_t)(f * (double)(n - 1)))] / 64; };
   std::ostringstream k;
-  k << c->name << "/cu" << c->n_cu << "/p" << o.precision << "/S" << o.segment << "/C" << o.ref_chunk << "/s"
+  k << c->name << "/cu" << c->n_cu << "/p" << o.precision << "/S" << o.segment << "/C" << o.ref_chunk << "/m" << (o.moves == 3 ? 3 : 2) << "/s"
     << c->n_strands << "/N" << bucket_log2(c->n_cols)
     << "/n" << bucket_log2((int64_t)n) << "/q" << q(0.1) << "," << q(0.5) << "," << q(0.9);
   return k.str();
@@ -248,7 +248,9 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
   };
 
   // stage 1: widths at one moderate chunk; stage 2: chunks at the best width
-  const int32_t chunk0 = o.ref_chunk ? o.ref_chunk : (int32_t)std::min<int64_t>(65536, cols);
+  // the three-move recurrence runs whole strands (its left move crosses chunks)
+  const bool m3 = o.moves == 3;
+  const int32_t chunk0 = m3 ? 0 : o.ref_chunk ? o.ref_chunk : (int32_t)std::min<int64_t>(65536, cols);
   double best_rate = -1.0;
   int32_t best_S = segs[0], best_chunk = chunk0;
   for (int32_t S : segs) {
@@ -264,7 +266,7 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
       best_S = S;
     }
   }
-  if (!o.ref_chunk) {
+  if (!o.ref_chunk && !m3) {
     for (int32_t ch : {1024, 4096, 16384, 131072, 262144}) {
       if (ch > 2 * cols) continue;  // longer than the sample: same as shorter ones
       double rate = 0;

@@ -200,8 +200,8 @@ int dtwax_normalize(dtwax_ctx* c, const float* d_raw, const int64_t* d_off, cons
 // ---- planning ------------------------------------------------------------
 }  // extern "C"
 
-int dtwax::occupancy(int precision, int S, int nstr, bool inj) {
-  return precision == DTWAX_F32 ? occupancy_f32(S, nstr, inj) : occupancy_f16(precision, S, inj);
+int dtwax::occupancy(int precision, int S, int nstr, bool m3) {
+  return precision == DTWAX_F32 ? occupancy_f32(S, nstr, m3) : occupancy_f16(precision, S, m3);
 }
 
 extern "C" {
@@ -242,6 +242,12 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   if (o.segment && !valid_segment(o.segment)) return fail(DTWAX_EINVAL, "segment must be 8, 16, 25 or 32");
   if (o.ref_chunk < 0) return fail(DTWAX_EINVAL, "ref_chunk < 0");
   if (o.compat_quirks) return fail(DTWAX_EINVAL, "compat_quirks is only available through dtwax_distances");
+  if (o.moves != 0 && o.moves != 2 && o.moves != 3) return fail(DTWAX_EINVAL, "moves must be 2 or 3");
+  const bool m3 = o.moves == 3;
+  if (m3 && o.precision != DTWAX_F16 && o.precision != DTWAX_F32)
+    return fail(DTWAX_EINVAL, "moves = 3 needs DTWAX_F16 or DTWAX_F32");
+  if (m3 && o.ref_chunk && o.ref_chunk < col_limit)
+    return fail(DTWAX_EINVAL, "moves = 3 needs whole-strand work items (ref_chunk 0)");
   std::vector<int32_t> lens(h_lens, h_lens + n_reads);
   int32_t qmax = 0;
   for (int32_t q : lens) {
@@ -255,6 +261,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   p->n_reads = n_reads;
   p->precision = o.precision;
   p->nstr = c->n_strands;
+  p->m3 = m3;
   // initial per-row drift; each band then adds its exact row minimum / 64 per
   // lane.  Measured against FP64 over Q = 1000..8000 (both classes): 0.01
   // keeps the max relative error below 1%; 0.1 over-subtracts for reads
@@ -303,7 +310,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   int64_t slots = 1, slots_sum = 0;
   for (int32_t r = 0; r < n_reads; r++) multiband |= lens[r] > 64 * seg_of[(size_t)r];
   for (int S : widths) {
-    const int64_t sl = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, false)) * c->n_cu;
+    const int64_t sl = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, m3)) * c->n_cu;
     slots = std::max(slots, sl);
     slots_sum += sl;
   }
@@ -311,7 +318,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   // chunking: explicit, or whole strands unless the GPU would be underfilled
   // or the band scratch would not fit the budget.
   int64_t chunk = o.ref_chunk ? o.ref_chunk : N;
-  if (!o.ref_chunk) {
+  if (!o.ref_chunk && !m3) {
     if ((int64_t)n_reads < slots && n_reads > 0) {
       int64_t want = (slots + n_reads - 1) / n_reads;
       int64_t ch = (N + want - 1) / want;
@@ -363,7 +370,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     g.S = S;
     g.item0 = (int32_t)hv.size();
     g.n_items = (int32_t)items.size();
-    const int64_t slots_S = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, false)) * c->n_cu;
+    const int64_t slots_S = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, m3)) * c->n_cu;
     g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(slots_S, (int64_t)items.size()));
     g.renorm_k = drift * (float)S;
     g.scratch_wg0 = scratch_rows;  // concurrent groups: disjoint band scratch
@@ -373,6 +380,21 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     p->groups.push_back(g);
   }
   p->n_items = (int32_t)hv.size();
+  if (m3 && multiband) {
+    // whole-strand band scratch rows: fewer resident waves if the rows of a
+    // full grid would not fit the scratch budget
+    const int64_t epc = (p->precision == DTWAX_F32) ? p->nstr : 1;
+    const int64_t row_bytes = (COL_PADL + max_w + COL_PADR) * epc * 4;
+    const int64_t cap = std::max<int64_t>(1, ((int64_t)8 << 30) / (row_bytes * (int64_t)p->groups.size()));
+    scratch_rows = 0;
+    max_grid = 1;
+    for (auto& g : p->groups) {
+      g.grid = (int)std::min<int64_t>(g.grid, cap);
+      g.scratch_wg0 = scratch_rows;
+      scratch_rows += g.grid;
+      max_grid = std::max(max_grid, g.grid);
+    }
+  }
   p->grid = max_grid;
 
   auto cleanup = [&](const std::string& m) {
@@ -503,7 +525,7 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
       k.counter = L.counter + g;
       k.renorm_k = G.renorm_k;
       k.scratch = L.scratch ? (void*)((float*)L.scratch + G.scratch_wg0 * L.scratch_stride * epc) : nullptr;
-      HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, std::max(1, std::min(G.grid, G.n_items)), false, gs));
+      HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, std::max(1, std::min(G.grid, G.n_items)), L.m3, gs));
     }
     if (gs != st) {
       HIP_TRY(hipEventRecord(G.done, gs));
@@ -539,6 +561,7 @@ static int run_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const
   L.groups = p->groups.data();
   L.n_groups = (int)p->groups.size();
   L.fork = p->fork;
+  L.m3 = p->m3;
   L.out = reinterpret_cast<DeviceResult*>(d_out);
   if (p->timing) {
     L.ev0 = p->ev0;

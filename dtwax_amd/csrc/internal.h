@@ -54,6 +54,7 @@ struct ScoreLaunch {
   void* state = nullptr;
   int64_t row_stride = 0;
   float renorm_k = 0.f;
+  bool m3 = false;  // three moves
   DeviceResult* out = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   const LaunchGroup* groups = nullptr;  // counter[g] for group g
@@ -89,6 +90,7 @@ struct dtwax_plan {
   dtwax_ctx* ctx = nullptr;
   int32_t n_reads = 0;
   int precision = DTWAX_F16;
+  bool m3 = false;  // three moves (whole-strand items)
   int S = 32;  // 0 when the reads are split over several segment widths
   std::vector<dtwax::LaunchGroup> groups;  // launched concurrently, one stream each
   hipEvent_t fork = nullptr;
@@ -111,7 +113,7 @@ struct dtwax_plan {
 
 namespace dtwax {
 int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st);
-int occupancy(int precision, int S, int nstr, bool inj);
+int occupancy(int precision, int S, int nstr, bool m3);
 // dtwax_plan_create over reference columns [0, col_limit) only
 int plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, const dtwax_score_opts* opts,
                int64_t col_limit, dtwax_plan** out);

@@ -58,7 +58,8 @@ struct KParams {
   float renorm_k;            // F16_RENORM per-lane drift constant (S rows' worth)
 };
 
-hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool inj,
+// m3: three moves (the left move D[r][c-1] too; DTWAX_F16 and DTWAX_F32 only)
+hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool m3,
                        hipStream_t st);
 // zero n_counters work counters (n_counters <= 256) and set n_keys keys to ~0
 hipError_t launch_reset(int32_t* counters, int32_t n_counters, unsigned long long* keys, int64_t n_keys,
@@ -66,8 +67,8 @@ hipError_t launch_reset(int32_t* counters, int32_t n_counters, unsigned long lon
 hipError_t launch_finalize(const unsigned long long* keys, int32_t n_reads, int32_t n_strands,
                            DeviceResult* out, hipStream_t st);
 hipError_t launch_selftest_lanes(int32_t* d_out, hipStream_t st);
-int occupancy_f16(int precision, int S, bool inj);
-int occupancy_f32(int S, int nstr, bool inj);
+int occupancy_f16(int precision, int S, bool m3);
+int occupancy_f32(int S, int nstr, bool m3);
 
 // normalisation (normalize_kernels.hip)
 hipError_t launch_normalize(const float* raw, const int64_t* off, const int32_t* len, int32_t n_reads,

@@ -62,6 +62,12 @@ __device__ __forceinline__ T dpp_shl1(T old, T src) {
 __device__ __forceinline__ h2 hsplat(half_t v) { return (h2){v, v}; }
 __device__ __forceinline__ h2 hmin2(h2 a, h2 b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ h2 hfma2(h2 a, h2 b, h2 c) { return __builtin_elementwise_fma(a, b, c); }
+// Three-input packed min: one v_pk_minimum3_f16 on gfx950.  IEEE minimum
+// (NaN-propagating, -0 < +0) equals min here: no NaN reaches it and every D
+// is a sum of squares, so no -0.
+__device__ __forceinline__ h2 hmin3(h2 a, h2 b, h2 c) {
+  return __builtin_elementwise_minimum(__builtin_elementwise_minimum(a, b), c);
+}
 
 // Order-preserving map of a float to uint32 (for the 64-bit min keys).
 __device__ __forceinline__ uint32_t ordered_bits(float f) {
@@ -153,18 +159,25 @@ __device__ __forceinline__ void f16_cells(const F16Band<S, MODE>& B, h2 s, h2 ti
 // column (s_next) into tc, which the next step starts from.
 constexpr int PG = 2;
 
-template <int S, int MODE>
+//
+// M3 (the reference built without -DNO_REF_DEL, DTW.cu:23-27): the left move
+// D[r][c-1] = D_i(old) joins the min, which becomes one three-input min in
+// the chain's min slot (still 4 ops per cell).
+template <int S, int MODE, bool M3>
 __device__ __forceinline__ void f16_cells_pipe(const F16Band<S, MODE>& B, h2 s, h2 s_next, h2 (&tc)[PG],
                                                h2 tin, h2 dg, const h2 (&Do)[S], h2 (&Dn)[S]) {
   h2 t[S], c[S];
 #pragma unroll
   for (int i = 0; i < PG; i++) t[i] = tc[i];
   c[0] = t[0] * t[0];
+  h2 dd = dg;
   __builtin_amdgcn_sched_barrier(0);
-  h2 tp = tin, dd = dg;
+  h2 tp = tin;
 #pragma unroll
   for (int i = 0; i < S; i++) {
-    const h2 m = hmin2(tp, dd);
+    h2 m;
+    if constexpr (M3) m = hmin3(tp, dd, Do[i]);
+    else m = hmin2(tp, dd);
     __builtin_amdgcn_sched_barrier(0);
     if (i + 1 < S) c[i + 1] = t[i + 1] * t[i + 1];
     __builtin_amdgcn_sched_barrier(0);
@@ -184,7 +197,6 @@ struct SweepIO {
   const h2* trow;  // top-boundary row (absolute columns) or null for a free start
   h2* orow;        // row output (absolute columns) or null
   int32_t x, c0, c1;
-  int32_t inj_lane;  // lane that takes the top row (first real lane)
   bool final_band;
   bool top_diag;     // lane 0 takes trow[x-1] as the diagonal of its first cell (resumed
                      // top row; needs no virtual rows in front of the band)
@@ -192,7 +204,7 @@ struct SweepIO {
 
 // Sweep one band across columns [x, c1) (+ ramp).  Accumulates the min/argmin
 // of the band's last row over [c0, c1) into best/pos when final_band.
-template <int S, int MODE, bool INJ>
+template <int S, int MODE, bool M3>
 __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const SweepIO& io, h2 K,
                                           h2* __restrict__ ring_s, h2* __restrict__ ring_t,
                                           int lane, h2& best, int32_t& posf, int32_t& posr,
@@ -229,7 +241,6 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
   h2 tin = INF, dg = INF, hist = INF;
   if (lane == 0) tin = ring_t[64] - K;  // top row at column x for lane 0
   if (lane == 0 && io.top_diag) dg = ring_t[63] - K;
-  if (INJ && lane == io.inj_lane) tin = ring_t[64 - lane] - K;
 
   constexpr bool PIPE = MODE != DTWAX_F16_FUSED;
   // pipelined forms: column of step 0 and its first PG row costs
@@ -257,7 +268,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
             // ahead (slot base + 65 at most: inside the mirrored ring)
             const h2 s_next = rs[u + 1];
             const h2 tf_next = rt[u + 2];
-            f16_cells_pipe<S, MODE>(B, s_cur, s_next, tc, tin, dg, D0, D1);
+            f16_cells_pipe<S, MODE, M3>(B, s_cur, s_next, tc, tin, dg, D0, D1);
             s_cur = s_next;
             tf = tf_cur;
             tf_cur = tf_next;
@@ -268,7 +279,6 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           }
           dg = tin;
           tin = dpp_shr1(tf, D1[S - 1]);
-          if (INJ) tin = (lane == io.inj_lane) ? tf : tin;
           if (MODE == DTWAX_F16_RENORM) tin = tin - K;
           hist = dpp_shl1(D1[S - 1], hist);
         }
@@ -277,7 +287,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           if constexpr (PIPE) {
             const h2 s_next = rs[u + 2];
             const h2 tf_next = rt[u + 3];
-            f16_cells_pipe<S, MODE>(B, s_cur, s_next, tc, tin, dg, D1, D0);
+            f16_cells_pipe<S, MODE, M3>(B, s_cur, s_next, tc, tin, dg, D1, D0);
             s_cur = s_next;
             tf = tf_cur;
             tf_cur = tf_next;
@@ -288,7 +298,6 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           }
           dg = tin;
           tin = dpp_shr1(tf, D0[S - 1]);
-          if (INJ) tin = (lane == io.inj_lane) ? tf : tin;
           if (MODE == DTWAX_F16_RENORM) tin = tin - K;
           hist = dpp_shl1(D0[S - 1], hist);
         }
@@ -319,7 +328,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
 
 __device__ __forceinline__ half_t clamp_half(float v) { return (half_t)v; }
 
-template <int S, int MODE, bool INJ>
+template <int S, int MODE, bool M3>
 __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* ring_s, h2* ring_t,
                          h2* scratch) {
   const int32_t read = it.read;
@@ -374,15 +383,14 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     io.x = it.x;
     io.c0 = it.c0;
     io.c1 = it.c1;
-    io.inj_lane = (b == 0) ? (P / S) : 0;
     io.top_diag = (b == 0) && (trow_ext != nullptr);
     h2 rowmin = hsplat((half_t)__builtin_inff());
     h2 Kl = Kb;
     if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = Kb + prev_min;
-    f16_sweep<S, MODE, INJ>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
+    f16_sweep<S, MODE, M3>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
     if (MODE == DTWAX_F16_RENORM) {
       // lanes whose subtraction lies on the path to the last row
-      const int32_t first = INJ ? io.inj_lane : 0;
+      const int32_t first = 0;
       const h2 k0 = __builtin_bit_cast(h2, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Kl), first));
       off_f += (float)(63 - first) * (float)Kb.x + (float)k0.x;
       off_r += (float)(63 - first) * (float)Kb.y + (float)k0.y;
@@ -410,7 +418,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
   publish_min(&p.keys[2 * (int64_t)read + 1], wave_min_u64(kr), lane);
 }
 
-template <int S, int MODE, bool INJ>
+template <int S, int MODE, bool M3>
 __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
   __shared__ h2 ring_s[RING_ALLOC];
   __shared__ h2 ring_t[RING_ALLOC];
@@ -420,14 +428,14 @@ __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
     const int idx = fetch_item(p.counter, lane);
     if (idx >= p.n_items) break;
     WorkItem it = p.items[idx];
-    f16_item<S, MODE, INJ>(p, it, lane, ring_s, ring_t, scratch);
+    f16_item<S, MODE, M3>(p, it, lane, ring_s, ring_t, scratch);
   }
 }
 
 // ------------------------------------------------------------------------
 // binary32 form (DTWAX_F32), NSTR strands (1: single sequence, 2: fwd/rev)
 // ------------------------------------------------------------------------
-template <int S>
+template <int S, bool M3>
 __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)[S], float s,
                                           float tin, float dg, const float (&Do)[S],
                                           float (&Dn)[S]) {
@@ -436,6 +444,7 @@ __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)
   for (int i = 0; i < S; i++) {
     float t = __builtin_fmaf(s, A[i], Qn[i]);  // == rn(s - q) for a real row
     float c = t * t;
+    if constexpr (M3) dd = __builtin_fminf(dd, Do[i]);  // left move
     float m = __builtin_fminf(tp, dd);
     float v = c + m;
     Dn[i] = v;
@@ -444,26 +453,25 @@ __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)
   }
 }
 
-template <int S, int NSTR, bool INJ>
+template <int S, int NSTR, bool M3>
 __device__ __forceinline__ void f32_half_step(const float (&A)[S], const float (&Qn)[S],
                                               const float* rs, const float* rt, int u,
                                               float (&tin)[NSTR], float (&dg)[NSTR],
                                               float (&hist)[NSTR], float (&Do)[NSTR][S],
-                                              float (&Dn)[NSTR][S], int lane, int inj) {
+                                              float (&Dn)[NSTR][S]) {
 #pragma unroll
   for (int st = 0; st < NSTR; st++) {
     const float s = rs[u * NSTR + st];
-    f32_cells<S>(A, Qn, s, tin[st], dg[st], Do[st], Dn[st]);
+    f32_cells<S, M3>(A, Qn, s, tin[st], dg[st], Do[st], Dn[st]);
     const float tf = rt[(u + 1) * NSTR + st];
     dg[st] = tin[st];
     float ti = dpp_shr1(tf, Dn[st][S - 1]);
-    if (INJ) ti = (lane == inj) ? tf : ti;
     tin[st] = ti;
     hist[st] = dpp_shl1(Dn[st][S - 1], hist[st]);
   }
 }
 
-template <int S, int NSTR, bool INJ>
+template <int S, int NSTR, bool M3>
 __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* ring_s,
                          float* ring_t, float* scratch) {
   const int32_t read = it.read;
@@ -501,7 +509,6 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
     const bool final_band = (b == nb - 1);
     float* orow = final_band ? orow_ext : scr;
     const int32_t x = it.x, c1 = it.c1, c0 = it.c0;
-    const int32_t inj = (b == 0) ? (P / S) : 0;
     const int32_t nsteps = ((c1 - x + 63) + 63) & ~63;
 
     for (int k = 0; k < 3; k++) {
@@ -532,7 +539,6 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
       hist[st] = INF;
       if (lane == 0) tin[st] = ring_t[64 * NSTR + st];
       if (lane == 0 && b == 0 && trow_ext) dg[st] = ring_t[63 * NSTR + st];
-      if (INJ && lane == inj) tin[st] = ring_t[(64 - lane) * NSTR + st];
     }
 
     for (int32_t T = 0; T < nsteps; T += 64) {
@@ -549,8 +555,8 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
         const float* rt = ring_t + (base + g) * NSTR;
 #pragma unroll
         for (int u = 0; u < 8; u += 2) {
-          f32_half_step<S, NSTR, INJ>(A, Qn, rs, rt, u, tin, dg, hist, D0, D1, lane, inj);
-          f32_half_step<S, NSTR, INJ>(A, Qn, rs, rt, u + 1, tin, dg, hist, D1, D0, lane, inj);
+          f32_half_step<S, NSTR, M3>(A, Qn, rs, rt, u, tin, dg, hist, D0, D1);
+          f32_half_step<S, NSTR, M3>(A, Qn, rs, rt, u + 1, tin, dg, hist, D1, D0);
         }
       }
       const int32_t col = x + T + lane - 63;
@@ -586,7 +592,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
   }
 }
 
-template <int S, int NSTR, bool INJ>
+template <int S, int NSTR, bool M3>
 __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
   __shared__ float ring_s[RING_ALLOC * NSTR];
   __shared__ float ring_t[RING_ALLOC * NSTR];
@@ -596,7 +602,7 @@ __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
     const int idx = fetch_item(p.counter, lane);
     if (idx >= p.n_items) break;
     WorkItem it = p.items[idx];
-    f32_item<S, NSTR, INJ>(p, it, lane, ring_s, ring_t, scratch);
+    f32_item<S, NSTR, M3>(p, it, lane, ring_s, ring_t, scratch);
   }
 }
 
@@ -645,50 +651,56 @@ __global__ void selftest_lanes_kernel(int32_t* out) {
 // ------------------------------------------------------------------------
 // launch table
 // ------------------------------------------------------------------------
+// m3 (three moves) exists for the exact binary16 form and binary32 only
 template <int S, int MODE>
-static hipError_t launch_f16(const KParams& p, int grid, bool inj, hipStream_t st) {
-  if (inj)
-    hipLaunchKernelGGL((sdtw_f16_kernel<S, MODE, true>), dim3(grid), dim3(64), 0, st, p);
-  else
-    hipLaunchKernelGGL((sdtw_f16_kernel<S, MODE, false>), dim3(grid), dim3(64), 0, st, p);
+static hipError_t launch_f16(const KParams& p, int grid, bool m3, hipStream_t st) {
+  if constexpr (MODE == DTWAX_F16) {
+    if (m3) {
+      hipLaunchKernelGGL((sdtw_f16_kernel<S, MODE, true>), dim3(grid), dim3(64), 0, st, p);
+      return hipGetLastError();
+    }
+  } else {
+    if (m3) return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL((sdtw_f16_kernel<S, MODE, false>), dim3(grid), dim3(64), 0, st, p);
   return hipGetLastError();
 }
 
 template <int S>
-static hipError_t launch_f32(const KParams& p, int grid, int nstr, bool inj, hipStream_t st) {
+static hipError_t launch_f32(const KParams& p, int grid, int nstr, bool m3, hipStream_t st) {
   if (nstr == 1) {
-    if (inj) hipLaunchKernelGGL((sdtw_f32_kernel<S, 1, true>), dim3(grid), dim3(64), 0, st, p);
+    if (m3) hipLaunchKernelGGL((sdtw_f32_kernel<S, 1, true>), dim3(grid), dim3(64), 0, st, p);
     else hipLaunchKernelGGL((sdtw_f32_kernel<S, 1, false>), dim3(grid), dim3(64), 0, st, p);
   } else {
-    if (inj) hipLaunchKernelGGL((sdtw_f32_kernel<S, 2, true>), dim3(grid), dim3(64), 0, st, p);
+    if (m3) hipLaunchKernelGGL((sdtw_f32_kernel<S, 2, true>), dim3(grid), dim3(64), 0, st, p);
     else hipLaunchKernelGGL((sdtw_f32_kernel<S, 2, false>), dim3(grid), dim3(64), 0, st, p);
   }
   return hipGetLastError();
 }
 
 template <int MODE>
-static hipError_t launch_f16_s(const KParams& p, int S, int grid, bool inj, hipStream_t st) {
+static hipError_t launch_f16_s(const KParams& p, int S, int grid, bool m3, hipStream_t st) {
   switch (S) {
-    case 8: return launch_f16<8, MODE>(p, grid, inj, st);
-    case 16: return launch_f16<16, MODE>(p, grid, inj, st);
-    case 25: return launch_f16<25, MODE>(p, grid, inj, st);
-    case 32: return launch_f16<32, MODE>(p, grid, inj, st);
+    case 8: return launch_f16<8, MODE>(p, grid, m3, st);
+    case 16: return launch_f16<16, MODE>(p, grid, m3, st);
+    case 25: return launch_f16<25, MODE>(p, grid, m3, st);
+    case 32: return launch_f16<32, MODE>(p, grid, m3, st);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool inj,
+hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool m3,
                        hipStream_t st) {
   switch (precision) {
-    case DTWAX_F16: return launch_f16_s<DTWAX_F16>(p, S, grid, inj, st);
-    case DTWAX_F16_FUSED: return launch_f16_s<DTWAX_F16_FUSED>(p, S, grid, inj, st);
-    case DTWAX_F16_RENORM: return launch_f16_s<DTWAX_F16_RENORM>(p, S, grid, inj, st);
+    case DTWAX_F16: return launch_f16_s<DTWAX_F16>(p, S, grid, m3, st);
+    case DTWAX_F16_FUSED: return launch_f16_s<DTWAX_F16_FUSED>(p, S, grid, m3, st);
+    case DTWAX_F16_RENORM: return launch_f16_s<DTWAX_F16_RENORM>(p, S, grid, m3, st);
     case DTWAX_F32:
       switch (S) {
-        case 8: return launch_f32<8>(p, grid, nstr, inj, st);
-        case 16: return launch_f32<16>(p, grid, nstr, inj, st);
-        case 25: return launch_f32<25>(p, grid, nstr, inj, st);
-        case 32: return launch_f32<32>(p, grid, nstr, inj, st);
+        case 8: return launch_f32<8>(p, grid, nstr, m3, st);
+        case 16: return launch_f32<16>(p, grid, nstr, m3, st);
+        case 25: return launch_f32<25>(p, grid, nstr, m3, st);
+        case 32: return launch_f32<32>(p, grid, nstr, m3, st);
       }
   }
   return hipErrorInvalidValue;
@@ -724,10 +736,11 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t s
   return hipGetLastError();
 }
 
-int occupancy_f16(int precision, int S, bool inj) {
+int occupancy_f16(int precision, int S, bool m3) {
   int n = 0;
 #define OCC(MODE, SS)                                                                          \
-  if (inj) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<SS, MODE, true>, 64, 0); \
+  if (m3 && MODE == DTWAX_F16)                                                                 \
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<SS, DTWAX_F16, true>, 64, 0); \
   else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<SS, MODE, false>, 64, 0);
 #define OCC_S(MODE)            \
   switch (S) {                 \
@@ -746,10 +759,10 @@ int occupancy_f16(int precision, int S, bool inj) {
   return n;
 }
 
-int occupancy_f32(int S, int nstr, bool inj) {
+int occupancy_f32(int S, int nstr, bool m3) {
   int n = 0;
 #define OCC(SS, NS)                                                                            \
-  if (inj) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<SS, NS, true>, 64, 0); \
+  if (m3) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<SS, NS, true>, 64, 0); \
   else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<SS, NS, false>, 64, 0);
   switch (S) {
     case 8: if (nstr == 1) { OCC(8, 1) } else { OCC(8, 2) } break;

@@ -81,7 +81,14 @@ typedef struct {
   int32_t compat_quirks; /* 1: reproduce the reference kernel's band-seam and tile-seam
                             deviations (DTW.cu:179-180, 315, 385-388, 442); needs the
                             reference geometry (see dtwax_distances) */
-  int32_t reserved[3];
+  int32_t moves;         /* 2 (or 0): D = cost + min(top, diag), the shipped -DNO_REF_DEL
+                            build; 3: + the left move D[r][c-1] (reference deletion, the
+                            build without -DNO_REF_DEL, DTW.cu:23-27).  3 needs DTWAX_F16
+                            or DTWAX_F32 and whole-strand work items (ref_chunk 0 or at
+                            least the reference length): a
+                            left move reaches back any number of columns, so column
+                            chunks with an overlap are no longer exact */
+  int32_t reserved[2];
 } dtwax_score_opts;
 
 /* ---- context ----------------------------------------------------------- */

@@ -62,6 +62,16 @@ def lib():
                                     C.c_int, _f32p, _i64p, C.c_int]
         L.orc_batch_f16.argtypes = [_u16p, _i64p, _i64p, C.c_int64, _u16p, C.c_int64,
                                     C.c_int, C.c_int, _u16p, _i64p, C.c_int]
+        L.orc_sdtw3_f64.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int64)]
+        L.orc_sdtw3_f32.argtypes = [_f32p, C.c_int64, _f32p, C.c_int64, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_int64)]
+        L.orc_sdtw3_f16.argtypes = [_u16p, C.c_int64, _u16p, C.c_int64, C.POINTER(C.c_uint16),
+                                    C.POINTER(C.c_int64)]
+        L.orc_batch3_f32.argtypes = [_f32p, _i64p, _i64p, C.c_int64, _f32p, C.c_int64,
+                                     C.c_int, _f32p, _i64p, C.c_int]
+        L.orc_batch3_f16.argtypes = [_u16p, _i64p, _i64p, C.c_int64, _u16p, C.c_int64,
+                                     C.c_int, _u16p, _i64p, C.c_int]
         L.orc_cpu_baseline_f32.restype = C.c_int
         L.orc_cpu_baseline_f32.argtypes = [_f32p, _i64p, _i64p, C.c_int64, _f32p,
                                            C.c_int64, C.c_int, _f32p, _i64p, C.c_int]
@@ -138,6 +148,52 @@ def sdtw_f16(q_h, s_h, init_row=None, want_row=False, fused=False, scalar=False)
           lib().orc_sdtw_f16_scalar if scalar else lib().orc_sdtw_f16)
     fn(q, q.size, s, s.size, _ptr(init), _ptr(row), C.byref(sc), C.byref(am))
     return (sc.value, am.value, row) if want_row else (sc.value, am.value)
+
+
+def sdtw3_f64(q, s):
+    """Three-move recurrence (left move too), FP64: (score, argmin)."""
+    q = np.ascontiguousarray(q, np.float64)
+    s = np.ascontiguousarray(s, np.float64)
+    sc, am = C.c_double(), C.c_int64()
+    lib().orc_sdtw3_f64(q, q.size, s, s.size, C.byref(sc), C.byref(am))
+    return sc.value, am.value
+
+
+def sdtw3_f32(q, s):
+    q = np.ascontiguousarray(q, np.float32)
+    s = np.ascontiguousarray(s, np.float32)
+    sc, am = C.c_float(), C.c_int64()
+    lib().orc_sdtw3_f32(q, q.size, s, s.size, C.byref(sc), C.byref(am))
+    return sc.value, am.value
+
+
+def sdtw3_f16(q_h, s_h):
+    """binary16 bit patterns in, (score bits, argmin) out."""
+    q = np.ascontiguousarray(q_h, np.uint16)
+    s = np.ascontiguousarray(s_h, np.uint16)
+    sc, am = C.c_uint16(), C.c_int64()
+    lib().orc_sdtw3_f16(q, q.size, s, s.size, C.byref(sc), C.byref(am))
+    return sc.value, am.value
+
+
+def batch3_f32(queries, strands, n_threads=0):
+    strands = np.ascontiguousarray(np.atleast_2d(strands), np.float32)
+    flat, offs, lens = _pack(queries, np.float32)
+    n, k = len(queries), strands.shape[0]
+    sc = np.empty(n * k, np.float32)
+    am = np.empty(n * k, np.int64)
+    lib().orc_batch3_f32(flat, offs, lens, n, strands.ravel(), strands.shape[1], k, sc, am, n_threads)
+    return sc.reshape(n, k), am.reshape(n, k)
+
+
+def batch3_f16(queries_h, strands_h, n_threads=0):
+    strands = np.ascontiguousarray(np.atleast_2d(strands_h), np.uint16)
+    flat, offs, lens = _pack(queries_h, np.uint16)
+    n, k = len(queries_h), strands.shape[0]
+    sc = np.empty(n * k, np.uint16)
+    am = np.empty(n * k, np.int64)
+    lib().orc_batch3_f16(flat, offs, lens, n, strands.ravel(), strands.shape[1], k, sc, am, n_threads)
+    return sc.reshape(n, k), am.reshape(n, k)
 
 
 def trace_f64(q, s, cols):

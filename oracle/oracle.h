@@ -10,6 +10,7 @@
  * Contents (each function cites the reference file:line it restates):
  *   binary16 conversion            orc_f32_to_f16 / orc_f16_to_f32
  *   clean sDTW recurrence          orc_sdtw_f64 / _f32 / _f16 / _f16_fused
+ *   three-move recurrence          orc_sdtw3_f64 / _f32 / _f16  (sdtw3.c)
  *   full-matrix brute force        orc_sdtw_bruteforce_f64
  *   literal reference kernel       orc_emulate_reference   (src/include/DTW.cu)
  *   numpy-order statistics         orc_numpy_sum / orc_python_znorm
@@ -93,6 +94,22 @@ int orc_cpu_baseline_f32(const float* q, const int64_t* offsets,
                          const int64_t* lens, int64_t n_reads, const float* s,
                          int64_t N, int n_strands, float* score,
                          int64_t* argmin, int n_threads);
+
+/* ---- three moves (reference built without -DNO_REF_DEL, DTW.cu:23-27) ---
+ *   D[r][c] = (s_c - q_r)^2 + min(D[r][c-1], D[r-1][c], D[r-1][c-1])
+ * free start, column -1 = +inf; same per-op rounding as the forms above.  */
+void orc_sdtw3_f64(const double* q, int64_t Q, const double* s, int64_t N, double* score,
+                   int64_t* argmin);
+void orc_sdtw3_f32(const float* q, int64_t Q, const float* s, int64_t N, float* score,
+                   int64_t* argmin);
+void orc_sdtw3_f16(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, uint16_t* score,
+                   int64_t* argmin);
+void orc_batch3_f32(const float* q, const int64_t* offsets, const int64_t* lens, int64_t n_reads,
+                    const float* s, int64_t N, int n_strands, float* score, int64_t* argmin,
+                    int n_threads);
+void orc_batch3_f16(const uint16_t* q, const int64_t* offsets, const int64_t* lens, int64_t n_reads,
+                    const uint16_t* s, int64_t N, int n_strands, uint16_t* score, int64_t* argmin,
+                    int n_threads);
 
 /* ---- literal emulator of the reference kernel --------------------------
  * src/include/DTW.cu:151-572 (DTW) with compute_segment (:43-146), launched
