@@ -310,6 +310,62 @@ def squiggle_build(seq: str, model_means, ref_len=None, k=6):
     return out.reshape(2, ref_len), skipped.value
 
 
+def read_fasta(path):
+    """Sequence of a single-entry FASTA, as load_reference::ref_loader reads
+    it (load_reference.hpp:164-212): lines after a '>' header are joined; an
+    entry whose sequence lines contain a space is dropped; with several
+    entries the last one is used.  Raises ValueError if there is none."""
+    name, content, seq = None, [], None
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\n").rstrip("\r")
+            if not line or line[0] == ">":
+                if name:
+                    seq = "".join(content)
+                name = line[1:] if line else None
+                content = []
+            elif name:
+                if " " in line:
+                    name, content = None, []
+                else:
+                    content.append(line)
+    if name:
+        seq = "".join(content)
+    if seq is None:
+        raise ValueError(f"no valid FASTA entry in {path}")
+    return seq
+
+
+def read_kmer_model(path):
+    """k-mer model "KMER mean sd" -> means[4**k] float32 indexed by 2-bit code
+    (A=0 C=1 G=2 T=3, first base most significant); load_reference.hpp:131-161,
+    the first line of a k-mer wins."""
+    m = {}
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) < 3:
+                continue
+            try:
+                mean, _ = float(p[1]), float(p[2])
+            except ValueError:
+                continue
+            m.setdefault(p[0], mean)
+    if not m:
+        raise ValueError(f"empty k-mer model {path}")
+    k = len(min(m))
+    means = np.zeros(4 ** k, np.float32)
+    codes = {"A": 0, "C": 1, "G": 2, "T": 3}
+    for kmer, v in m.items():
+        if len(kmer) != k or any(ch not in codes for ch in kmer):
+            continue
+        c = 0
+        for ch in kmer:
+            c = c * 4 + codes[ch]
+        means[c] = np.float32(v)
+    return means
+
+
 def selftest_lanes():
     out = np.empty(128, np.int32)
     _check(lib().dtwax_selftest_lanes(out.ctypes.data))
