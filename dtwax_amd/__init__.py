@@ -30,7 +30,7 @@ class DtwaxError(RuntimeError):
 class ScoreOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("segment", C.c_int32), ("ref_chunk", C.c_int32),
                 ("renorm_drift", C.c_float), ("compat_quirks", C.c_int32),
-                ("moves", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("moves", C.c_int32), ("waves", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 RESULT_DTYPE = np.dtype([("score", np.float32, 2), ("argmin", np.int32, 2)])
@@ -168,11 +168,11 @@ def pack_reads(reads):
     return np.ascontiguousarray(flat, np.float32), offs, lens
 
 
-def make_opts(precision=F16, segment=0, ref_chunk=0, renorm_drift=0.0, moves=2):
+def make_opts(precision=F16, segment=0, ref_chunk=0, renorm_drift=0.0, moves=2, waves=0):
     if isinstance(precision, str):
         precision = PRECISIONS[precision]
     return ScoreOpts(int(precision), int(segment), int(ref_chunk), float(renorm_drift), 0,
-                     int(moves))
+                     int(moves), int(waves))
 
 
 class Plan:
@@ -278,8 +278,9 @@ class Context:
 
     def autotune(self, d_queries, d_offsets, lens, sample_reads=0, sample_cols=0, cache_path=None,
                  **kw):
-        """Times candidate (segment, ref_chunk) plans on a sample of the
-        workload (device queries/offsets, host lengths).  Returns
+        """Times the segment widths on a sample of the workload (device
+        queries/offsets, host lengths) and takes the plan's chunk rule for the
+        whole workload at the fastest width.  Returns
         (dict(segment=, ref_chunk=), table of timed candidates)."""
         lens = np.ascontiguousarray(lens, np.int32)
         base = make_opts(**kw)

@@ -7,14 +7,20 @@
 // of 64*S, and the per-step overhead is amortised over S rows) and reference
 // chunk (work items per read; each chunk after the first recomputes Q-1
 // overlap columns, while too few items starve the persistent grid) depend on
-// the length distribution and the reference size, so they are measured:
+// the length distribution and the reference size.  The width is measured:
 // candidate plans run over a sample of reads (spread evenly over the sorted
-// lengths, replicated until each candidate has 3 items per resident wave)
-// against the first sample_cols reference columns -- widths first, then
-// chunks at the best width -- the fastest per counted cell wins, and the
-// choice is cached per device, precision, strand count, reference-size and
-// length-distribution key (in memory, and in a text file when a cache path
-// is given).
+// lengths, replicated until each candidate has 6 items per resident wave)
+// against the first sample_cols reference columns at one moderate chunk, and
+// the fastest per counted cell wins.  The chunk is not ranked on the sample:
+// a sample holds a few items per wave, so its time is set by how its last
+// items land, which favours short chunks the full workload does not need
+// (measured on 16,384 reads x 5 Mb: 65536-column chunks won the sample,
+// 524288 runs the full workload 6% faster).  It is the plan's own rule at
+// the full workload's size with the chosen width: whole strands, or the
+// longest chunks whose band scratch fits the budget.  The choice is cached
+// per device, precision, strand count, reference-size and length-
+// distribution key (in memory, and in a text file when a cache path is
+// given).
 
 #include <hip/hip_runtime.h>
 
@@ -178,8 +184,10 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
   if (o.segment) segs = {o.segment};
   // Replicated sample: the full workload has far more work items than the
   // sample, so a candidate is timed on enough copies of the sample reads to
-  // give it at least 3 items per resident wave (steady state, not the tail).
-  constexpr int32_t kMaxReps = 32;
+  // give it at least kItemsPerWave items per resident wave (steady state,
+  // not the tail).
+  constexpr int32_t kMaxReps = 64;
+  constexpr int64_t kItemsPerWave = 6;
   std::vector<int64_t> r_off((size_t)m * kMaxReps);
   std::vector<int32_t> r_len((size_t)m * kMaxReps);
   for (int32_t k = 0; k < kMaxReps; k++)
@@ -215,8 +223,9 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
     if (rc) return rc;
     int64_t grid_sum = 0;
     for (const auto& g : p->groups) grid_sum += g.grid;
-    if (p->n_items > 0 && p->n_items < 3 * grid_sum) {
-      const int32_t reps = (int32_t)std::min<int64_t>(kMaxReps, (3 * grid_sum + p->n_items - 1) / p->n_items);
+    if (p->n_items > 0 && p->n_items < kItemsPerWave * grid_sum) {
+      const int32_t reps =
+          (int32_t)std::min<int64_t>(kMaxReps, (kItemsPerWave * grid_sum + p->n_items - 1) / p->n_items);
       if (reps > 1) {
         dtwax_plan_destroy(p);
         p = nullptr;
@@ -247,8 +256,8 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
     return DTWAX_OK;
   };
 
-  // stage 1: widths at one moderate chunk; stage 2: chunks at the best width
-  // the three-move recurrence runs whole strands (its left move crosses chunks)
+  // widths, timed at one moderate chunk (the three-move recurrence runs whole
+  // strands: its left move crosses chunks)
   const bool m3 = o.moves == 3;
   const int32_t chunk0 = m3 ? 0 : o.ref_chunk ? o.ref_chunk : (int32_t)std::min<int64_t>(65536, cols);
   double best_rate = -1.0;
@@ -266,24 +275,19 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
       best_S = S;
     }
   }
-  if (!o.ref_chunk && !m3) {
-    for (int32_t ch : {1024, 4096, 16384, 131072, 262144}) {
-      if (ch > 2 * cols) continue;  // longer than the sample: same as shorter ones
-      double rate = 0;
-      int rc = time_candidate(best_S, ch, rate);
-      if (rc) {
-        free_rep();
-        cleanup();
-        return rc;
-      }
-      if (rate > best_rate) {
-        best_rate = rate;
-        best_chunk = ch;
-      }
-    }
-  }
   free_rep();
   cleanup();
+  if (!o.ref_chunk && !m3) {
+    // the plan's chunk rule on the full workload with the chosen width
+    dtwax_score_opts t = o;
+    t.segment = best_S;
+    t.ref_chunk = 0;
+    dtwax_plan* p = nullptr;
+    int rc = plan_build(c, h_lens, n_reads, &t, c->n_cols, &p, /*dry=*/true);
+    if (rc) return rc;
+    best_chunk = p->chunk;
+    dtwax_plan_destroy(p);
+  }
   if (n_table) *n_table = std::min(nt, table_cap);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);

@@ -206,6 +206,16 @@ int dtwax::occupancy(int precision, int S, int nstr, bool m3) {
 
 extern "C" {
 
+// Bytes of band scratch a plan may take: a quarter of the free device
+// memory, at most 32 GB (on a 288 GB device: 512k-column chunks for a full
+// grid of 8000-sample reads, whose Q - 1 overlap re-computes < 2% of the
+// cells), at least 1 GB.
+static int64_t scratch_budget() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)32 << 30;
+  return std::max<int64_t>((int64_t)1 << 30, std::min<int64_t>((int64_t)32 << 30, (int64_t)(free_b / 4)));
+}
+
 static bool valid_segment(int S) { return S == 8 || S == 16 || S == 25 || S == 32; }
 
 static int auto_segment(const std::vector<int32_t>& lens) {
@@ -231,7 +241,7 @@ int dtwax_plan_create(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
 }  // extern "C"
 
 int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, const dtwax_score_opts* opts,
-                      int64_t col_limit, dtwax_plan** out) {
+                      int64_t col_limit, dtwax_plan** out, bool dry) {
   if (!c || !out || (!h_lens && n_reads)) return fail(DTWAX_EINVAL, "NULL argument");
   if (col_limit < 1 || col_limit > c->n_cols) return fail(DTWAX_EINVAL, "bad column limit");
   if (n_reads < 0) return fail(DTWAX_EINVAL, "n_reads < 0");
@@ -241,6 +251,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   if (o.precision < DTWAX_F16 || o.precision > DTWAX_F32) return fail(DTWAX_EINVAL, "bad precision");
   if (o.segment && !valid_segment(o.segment)) return fail(DTWAX_EINVAL, "segment must be 8, 16, 25 or 32");
   if (o.ref_chunk < 0) return fail(DTWAX_EINVAL, "ref_chunk < 0");
+  if (o.waves < 0) return fail(DTWAX_EINVAL, "waves < 0");
   if (o.compat_quirks) return fail(DTWAX_EINVAL, "compat_quirks is only available through dtwax_distances");
   if (o.moves != 0 && o.moves != 2 && o.moves != 3) return fail(DTWAX_EINVAL, "moves must be 2 or 3");
   const bool m3 = o.moves == 3;
@@ -325,7 +336,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
       chunk = std::max<int64_t>(ch, std::min<int64_t>(N, 8 * (int64_t)qmax));
     }
     if (multiband) {
-      const int64_t budget = (int64_t)8 << 30;  // bytes of band scratch
+      const int64_t budget = scratch_budget();
       const int64_t bytes_per_col = 4 * (p->precision == DTWAX_F32 ? p->nstr : 1);
       const int64_t max_w = budget / (std::min<int64_t>(slots_sum, (int64_t)n_reads * ((N + chunk - 1) / chunk)) * bytes_per_col);
       if (chunk + qmax > max_w) chunk = std::max<int64_t>(4096, max_w - qmax);
@@ -372,6 +383,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     g.n_items = (int32_t)items.size();
     const int64_t slots_S = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, m3)) * c->n_cu;
     g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(slots_S, (int64_t)items.size()));
+    if (o.waves > 0) g.grid = std::min(g.grid, o.waves);
     g.renorm_k = drift * (float)S;
     g.scratch_wg0 = scratch_rows;  // concurrent groups: disjoint band scratch
     scratch_rows += g.grid;
@@ -385,7 +397,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     // full grid would not fit the scratch budget
     const int64_t epc = (p->precision == DTWAX_F32) ? p->nstr : 1;
     const int64_t row_bytes = (COL_PADL + max_w + COL_PADR) * epc * 4;
-    const int64_t cap = std::max<int64_t>(1, ((int64_t)8 << 30) / (row_bytes * (int64_t)p->groups.size()));
+    const int64_t cap = std::max<int64_t>(1, scratch_budget() / (row_bytes * (int64_t)p->groups.size()));
     scratch_rows = 0;
     max_grid = 1;
     for (auto& g : p->groups) {
@@ -396,6 +408,10 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     }
   }
   p->grid = max_grid;
+  if (dry) {
+    *out = p;
+    return DTWAX_OK;
+  }
 
   auto cleanup = [&](const std::string& m) {
     dtwax_plan_destroy(p);

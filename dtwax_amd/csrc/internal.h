@@ -96,7 +96,6 @@ struct dtwax_plan {
   hipEvent_t fork = nullptr;
   int nstr = 2;
   int32_t chunk = 0;
-  bool inj = false;
   int grid = 0;
   int32_t n_items = 0;
   int64_t cells = 0, computed = 0;
@@ -114,7 +113,9 @@ struct dtwax_plan {
 namespace dtwax {
 int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st);
 int occupancy(int precision, int S, int nstr, bool m3);
-// dtwax_plan_create over reference columns [0, col_limit) only
+// dtwax_plan_create over reference columns [0, col_limit) only.  dry: the
+// plan's layout only (widths, chunk, items, grid, cell counts); nothing is
+// allocated on the device and the plan cannot run.
 int plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, const dtwax_score_opts* opts,
-               int64_t col_limit, dtwax_plan** out);
+               int64_t col_limit, dtwax_plan** out, bool dry = false);
 }  // namespace dtwax

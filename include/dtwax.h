@@ -88,7 +88,14 @@ typedef struct {
                             least the reference length): a
                             left move reaches back any number of columns, so column
                             chunks with an overlap are no longer exact */
-  int32_t reserved[2];
+  int32_t waves;         /* resident waves per launch (per width group): 0 = automatic (the
+                            kernel's occupancy x compute units, at most one per work
+                            item); a smaller value caps it.  Below full occupancy the
+                            hardware may place the waves unevenly over the SIMDs (a
+                            4096-read batch at 2 waves per SIMD ran 1.3% faster alone
+                            and 26% slower right after another kernel), so automatic
+                            is the safe choice */
+  int32_t reserved[1];
 } dtwax_score_opts;
 
 /* ---- context ----------------------------------------------------------- */
@@ -181,12 +188,15 @@ int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
  * 60-98) by timing the sDTW kernel on a sample: sample_reads reads (<= 0:
  * 512) spread over the sorted lengths, against the first sample_cols
  * reference columns (<= 0: 2^20), replicated until every candidate has at
- * least 3 work items per resident wave (steady state, as in the full run).
- * Stage 1 times segment in {0 = per-read widths, 8, 16, 25, 32} ({8, 16} for
- * DTWAX_F16_RENORM) at ref_chunk min(65536, sample_cols); stage 2 times
- * ref_chunk in {1024, 4096, 16384, 131072, 262144} at the best segment.  A
- * nonzero base->segment or base->ref_chunk fixes that one.  The fastest per
- * counted cell goes into *best (base's other fields kept).
+ * least 6 work items per resident wave (steady state, as in the full run).
+ * It times segment in {0 = per-read widths, 8, 16, 25, 32} ({8, 16} for
+ * DTWAX_F16_RENORM) at ref_chunk min(65536, sample_cols); the fastest per
+ * counted cell wins.  ref_chunk is then the plan's own rule for the whole
+ * workload at that width (whole strands, or the longest chunks whose band
+ * scratch fits the budget: a quarter of free device memory, at most 32 GB),
+ * not a sample timing, which would favour short chunks.  A nonzero
+ * base->segment or base->ref_chunk fixes that one.  The choice goes into
+ * *best (base's other fields kept).
  * d_queries/d_offsets: the workload's device queries (as for dtwax_score);
  * h_lens: host lengths.  The choice is cached per device, options, strand
  * count, reference-size and length-distribution key, in memory and, if

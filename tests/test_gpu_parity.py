@@ -256,10 +256,14 @@ def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     d_q, d_off = D.DeviceArray.from_host(flat), D.DeviceArray.from_host(offs)
     cache = tmp_path / "tune.txt"
     best, table = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, cache_path=cache)
-    assert len(table) >= 8
+    assert len(table) == 5  # widths 0 (per read), 8, 16, 25, 32
     assert best["segment"] in (0, 8, 16, 25, 32) and best["ref_chunk"] > 0
     top = table[np.argmax(table["gcups"])]
-    assert (top["segment"], top["ref_chunk"]) == (best["segment"], best["ref_chunk"])
+    assert top["segment"] == best["segment"]
+    # the chunk is the plan's own rule for the whole workload at that width
+    rule = ctx.plan(lns, precision="f16", segment=best["segment"])
+    assert rule.info()["ref_chunk"] == best["ref_chunk"]
+    rule.close()
     assert cache.read_text().split()[-2:] == [str(best["segment"]), str(best["ref_chunk"])]
     again, t2 = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, cache_path=cache)
     assert again == best and len(t2) == 0
@@ -280,3 +284,17 @@ def test_plan_and_score_argument_errors(ctx):
     q = [rng.standard_normal(50).astype(np.float32)]
     sc, am = ctx.score(q, precision="f16", segment=8)
     assert sc.shape == (1, 2) and np.all(np.isfinite(sc))
+
+
+def test_waves_cap_is_exact(ctx):
+    """Fewer resident waves (each pulling more work items) change nothing."""
+    rng = np.random.default_rng(93)
+    ref = rng.standard_normal((2, 3000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in (5, 700, 2100, 1300, 64, 4000)]
+    base = ctx.score(reads, precision="f16", ref_chunk=500)
+    for waves in (1, 3, 17):
+        got = ctx.score(reads, precision="f16", ref_chunk=500, waves=waves)
+        assert np.array_equal(base[0], got[0]) and np.array_equal(base[1], got[1])
+    with pytest.raises(D.DtwaxError, match="waves"):
+        ctx.plan(np.array([10], np.int32), waves=-1)
