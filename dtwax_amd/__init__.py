@@ -91,6 +91,7 @@ def lib():
             "dtwax_ru_submit": ([vp, i32, i32, vp], C.c_int),
             "dtwax_ru_poll": ([vp, vp, i32, C.POINTER(i32), i32], C.c_int),
             "dtwax_ru_stop": ([vp], C.c_int),
+            "dtwax_vbz_decode": ([vp, i64, vp, i32, vp, i64, C.POINTER(i64)], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -112,6 +113,7 @@ def exported_symbols():
         "dtwax_plan_set_timing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
         "dtwax_autotune", "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
         "dtwax_ru_release", "dtwax_ru_start", "dtwax_ru_submit", "dtwax_ru_poll", "dtwax_ru_stop",
+        "dtwax_vbz_decode",
     ]
 
 
@@ -309,6 +311,18 @@ def squiggle_build(seq: str, model_means, ref_len=None, k=6):
     _check(lib().dtwax_squiggle_build(seq.encode(), len(seq), k, mm.ctypes.data, ref_len,
                                       out.ctypes.data, C.byref(skipped)))
     return out.reshape(2, ref_len), skipped.value
+
+
+def vbz_decode(chunk, n_max, cd_values=(0, 2, 1, 1)):
+    """One VBZ-compressed FAST5 signal chunk (bytes) -> int16 samples.
+    cd_values: the filter's (version, integer_size, delta_zigzag, zstd_level)."""
+    buf = np.frombuffer(bytes(chunk), np.uint8)
+    cd = np.ascontiguousarray(cd_values, np.uint32)
+    out = np.empty(max(1, n_max), np.int16)
+    n = C.c_int64()
+    _check(lib().dtwax_vbz_decode(buf.ctypes.data, buf.size, cd.ctypes.data, cd.size,
+                                  out.ctypes.data, n_max, C.byref(n)))
+    return out[: n.value].copy()
 
 
 def read_fasta(path):

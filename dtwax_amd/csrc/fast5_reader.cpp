@@ -7,9 +7,11 @@
 //   * /<group>/Raw/Signal is read as float with no pA conversion (:137-194,
 //     conversion commented out at :181-186);
 //   * channel parameters come from /<group>/channel_id attributes (:279-310);
-//   * a signal that cannot be decoded because the VBZ filter (id 32020) is
-//     missing is reported (:170-173).  The reference exits the process there;
-//     this reader reports the file and moves on to the next one.
+//   * a signal compressed with the VBZ filter (id 32020) is read through the
+//     HDF5 plugin when it is loaded; without it the reference exits
+//     (:170-173), while this reader reads the stored chunks as they are
+//     (H5Dread_chunk) and decodes them with dtwax_vbz_decode; a signal it
+//     cannot decode either is reported and the next read follows.
 
 #include "fast5_reader.h"
 
@@ -17,7 +19,11 @@
 #include <dlfcn.h>
 #include <hdf5.h>
 
+#include <algorithm>
 #include <cstdlib>
+#include <cstring>
+
+#include "dtwax.h"
 
 namespace dtwax_io {
 
@@ -35,6 +41,8 @@ struct H5Api {
   H5FN(H5Aexists) H5FN(H5Aopen) H5FN(H5Aread) H5FN(H5Aclose) H5FN(H5Dopen2) H5FN(H5Dclose)
   H5FN(H5Dget_space) H5FN(H5Sget_simple_extent_dims) H5FN(H5Sclose) H5FN(H5Dread)
   H5FN(H5Dget_create_plist) H5FN(H5Pget_nfilters) H5FN(H5Pget_filter2) H5FN(H5Pclose)
+  // raw chunk access (HDF5 >= 1.10.5; optional: VBZ without the plugin)
+  H5FN(H5Dget_num_chunks) H5FN(H5Dget_chunk_info) H5FN(H5Dread_chunk) H5FN(H5Pget_chunk)
 #undef H5FN
   hid_t* native_float = nullptr;
   hid_t* native_double = nullptr;
@@ -64,6 +72,10 @@ H5Api& h5() {
     H5LOAD(H5Dread) H5LOAD(H5Dget_create_plist) H5LOAD(H5Pget_nfilters) H5LOAD(H5Pget_filter2)
     H5LOAD(H5Pclose)
 #undef H5LOAD
+    a.H5Dget_num_chunks = reinterpret_cast<decltype(a.H5Dget_num_chunks)>(dlsym(h, "H5Dget_num_chunks"));
+    a.H5Dget_chunk_info = reinterpret_cast<decltype(a.H5Dget_chunk_info)>(dlsym(h, "H5Dget_chunk_info"));
+    a.H5Dread_chunk = reinterpret_cast<decltype(a.H5Dread_chunk)>(dlsym(h, "H5Dread_chunk"));
+    a.H5Pget_chunk = reinterpret_cast<decltype(a.H5Pget_chunk)>(dlsym(h, "H5Pget_chunk"));
     a.native_float = reinterpret_cast<hid_t*>(dlsym(h, "H5T_NATIVE_FLOAT_g"));
     a.native_double = reinterpret_cast<hid_t*>(dlsym(h, "H5T_NATIVE_DOUBLE_g"));
     if (!all || !a.native_float || !a.native_double) {
@@ -111,20 +123,93 @@ static double read_attr_double(hid_t obj, const char* name, bool& ok) {
   return v;
 }
 
-static bool is_vbz(hid_t dset) {
+// The dataset's filter pipeline: whether it holds the VBZ filter, and
+// whether that is its only filter (with its stored values).
+struct VbzInfo {
+  bool vbz = false;
+  bool only = false;
+  unsigned cd[8] = {0, 2, 1, 1, 0, 0, 0, 0};
+  size_t n_cd = 0;
+  hsize_t chunk = 0;  // chunk length (1-D)
+};
+
+static VbzInfo vbz_info(hid_t dset) {
+  VbzInfo v;
   hid_t plist = h5().H5Dget_create_plist(dset);
-  if (plist < 0) return false;
-  bool found = false;
+  if (plist < 0) return v;
   int n = h5().H5Pget_nfilters(plist);
   for (int i = 0; i < n; i++) {
     unsigned int flags = 0;
-    size_t nelem = 0;
+    size_t nelem = 8;
+    unsigned cd[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     char name[64];
-    H5Z_filter_t id = h5().H5Pget_filter2(plist, (unsigned)i, &flags, &nelem, nullptr, sizeof(name), name, nullptr);
-    if (id == 32020) found = true;
+    H5Z_filter_t id = h5().H5Pget_filter2(plist, (unsigned)i, &flags, &nelem, cd, sizeof(name), name, nullptr);
+    if (id == 32020) {
+      v.vbz = true;
+      v.n_cd = nelem < 8 ? nelem : 8;
+      for (size_t k = 0; k < v.n_cd; k++) v.cd[k] = cd[k];
+    }
+  }
+  v.only = v.vbz && n == 1;
+  if (h5().H5Pget_chunk) {
+    hsize_t dims[1] = {0};
+    if (h5().H5Pget_chunk(plist, 1, dims) == 1) v.chunk = dims[0];
   }
   h5().H5Pclose(plist);
-  return found;
+  return v;
+}
+
+// Reads a 1-D int16 VBZ dataset chunk by chunk without the filter plugin.
+static bool read_vbz_native(hid_t d, const VbzInfo& v, hsize_t n, std::vector<float>& out, std::string& why) {
+  H5Api& a = h5();
+  if (!a.H5Dget_num_chunks || !a.H5Dget_chunk_info || !a.H5Dread_chunk || !a.H5Pget_chunk) {
+    why = "the HDF5 library has no raw chunk access (1.10.5 or newer needed)";
+    return false;
+  }
+  if (!v.only || v.chunk == 0) {
+    why = "the signal's filter pipeline holds more than the VBZ filter";
+    return false;
+  }
+  hsize_t nchunks = 0;
+  const hid_t space = a.H5Dget_space(d);
+  struct SpaceGuard {
+    hid_t s;
+    ~SpaceGuard() { if (s >= 0) h5().H5Sclose(s); }
+  } guard{space};
+  if (space < 0 || a.H5Dget_num_chunks(d, space, &nchunks) < 0) {
+    why = "cannot count the signal's chunks";
+    return false;
+  }
+  const uint32_t cd[4] = {v.cd[0], v.cd[1], v.cd[2], v.cd[3]};
+  std::vector<uint8_t> raw;
+  std::vector<int16_t> dec((size_t)v.chunk);
+  out.assign((size_t)n, 0.0f);
+  for (hsize_t c = 0; c < nchunks; c++) {
+    hsize_t off[1] = {0}, size = 0;
+    unsigned mask = 0;
+    haddr_t addr = 0;
+    if (a.H5Dget_chunk_info(d, space, c, off, &mask, &addr, &size) < 0) {
+      why = "cannot locate a chunk of the signal";
+      return false;
+    }
+    raw.resize((size_t)size);
+    uint32_t filters = 0;
+    if (a.H5Dread_chunk(d, H5P_DEFAULT, off, &filters, raw.data()) < 0) {
+      why = "cannot read a chunk of the signal";
+      return false;
+    }
+    int64_t got = 0;
+    if (filters & 1u) {  // the filter was skipped for this chunk: stored as is
+      got = (int64_t)std::min<size_t>(raw.size() / 2, dec.size());
+      std::memcpy(dec.data(), raw.data(), (size_t)got * 2);
+    } else if (dtwax_vbz_decode(raw.data(), (int64_t)raw.size(), cd, (int32_t)std::min<size_t>(v.n_cd, 4),
+                                dec.data(), (int64_t)dec.size(), &got) != DTWAX_OK) {
+      why = dtwax_last_error();
+      return false;
+    }
+    for (int64_t i = 0; i < got && off[0] + (hsize_t)i < n; i++) out[(size_t)(off[0] + i)] = (float)dec[(size_t)i];
+  }
+  return true;
 }
 
 bool read_fast5_file(const std::string& path, std::vector<Fast5Read>& out, std::string& err) {
@@ -190,8 +275,13 @@ bool read_fast5_file(const std::string& path, std::vector<Fast5Read>& out, std::
     r.samples.resize((size_t)n);
     if (n && h5().H5Dread(d, *h5().native_float, H5S_ALL, H5S_ALL, H5P_DEFAULT, r.samples.data()) < 0) {
       r.samples.clear();
-      if (is_vbz(d))
-        err = path + ": the fast5 file is compressed with VBZ but the required HDF5 plugin is not loaded";
+      const VbzInfo v = vbz_info(d);
+      std::string why;
+      if (v.vbz && !read_vbz_native(d, v, n, r.samples, why)) {
+        r.samples.clear();
+        err = path + ": the fast5 file is compressed with VBZ, the HDF5 plugin is not loaded and "
+                     "the built-in decoder failed: " + why;
+      }
     }
     h5().H5Sclose(sp);
     h5().H5Dclose(d);

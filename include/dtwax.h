@@ -17,6 +17,9 @@
  *                                                         src/main.cu:78-121
  *   dtwax_squiggle_build   load_reference::load_ref_coeffs src/include/load_reference.hpp:33-108
  *   dtwax_ru_*             the Read Until loop            src/ru_main.cu:172-326
+ *   dtwax_vbz_decode       the VBZ HDF5 plugin's decode   fast5/nanopolish_fast5_io.cpp:162-176
+ *                                                         (vbz_compression v1.0.1,
+ *                                                         install_hdf5_vbz.sh:43-58)
  */
 #ifndef DTWAX_H
 #define DTWAX_H
@@ -139,6 +142,21 @@ int dtwax_reference_upload(dtwax_ctx* ctx, const float* levels, int32_t n_strand
 int dtwax_squiggle_build(const char* seq, int64_t seq_len, int32_t k,
                          const float* model_means, int64_t ref_len, float* out,
                          int64_t* n_skipped);
+
+/* Decode one chunk of a FAST5 Raw/Signal dataset compressed with ONT's VBZ
+ * filter (HDF5 filter 32020), for files read without the VBZ plugin.
+ * cd_values: the filter's stored values (version, integer_size,
+ * delta_zigzag, zstd_level; missing ones default to 0, 2, 1, 1).  The chunk
+ * is [uint32 LE decoded byte count][zstd frame (zstd_level > 0) of a
+ * stream-VByte block of the zig-zag sample differences]; version 0 blocks
+ * have 2-bit keys for 1/2/3/4 bytes, version 1 blocks 2-bit keys for 0/1/2/4
+ * bytes or 1-bit keys for 1/2 bytes, recognised from the block (a block
+ * that fits none, or both, is an error).  integer_size must be 2.  Writes
+ * n = decoded bytes / 2 int16 samples (n <= dst_cap) to dst and sets
+ * *n_out.  zstd is loaded at run time (libzstd.so.1 or DTWAX_ZSTD_LIB).
+ * Host-side. */
+int dtwax_vbz_decode(const void* src, int64_t src_size, const uint32_t* cd_values,
+                     int32_t n_cd_values, int16_t* dst, int64_t dst_cap, int64_t* n_out);
 
 /* ---- normalisation -----------------------------------------------------
  * Normalise n_reads device-resident reads (float samples; read i starts at
