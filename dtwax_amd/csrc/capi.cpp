@@ -219,16 +219,25 @@ static int64_t scratch_budget() {
 static bool valid_segment(int S) { return S == 8 || S == 16 || S == 25 || S == 32; }
 
 static int auto_segment(const std::vector<int32_t>& lens) {
-  // minimise padded rows x per-step overhead (4*S cell ops + ~4 others)
+  // binary32: one width per batch, the least padded rows x the measured time
+  // per computed cell of that width relative to S = 16 (1.147 at 8, 1.058 at
+  // 25, 1.088 at 32: S = 32 holds 2 waves per SIMD, S = 16 three) x 1.5% per
+  // extra band
   static const int cand[] = {8, 16, 25, 32};
+  static const double f0[] = {1.147, 1.0, 1.058, 1.088};
   double best = 1e300;
-  int bs = 32;
-  for (int S : cand) {
-    const int64_t BR = 64 * S;
-    double rows = 0;
-    for (int32_t q : lens) rows += (double)(((int64_t)q + BR - 1) / BR * BR);
-    const double cost = rows * (4.0 * S + 4.0) / (4.0 * S);
-    if (cost < best - 1e-9) { best = cost; bs = S; }
+  int bs = 16;
+  for (int ci = 0; ci < 4; ci++) {
+    const int64_t BR = 64 * cand[ci];
+    double cost = 0;
+    for (int32_t q : lens) {
+      const int64_t nb = ((int64_t)q + BR - 1) / BR;
+      cost += (double)(nb * BR) * f0[ci] * (1.0 + 0.015 * (double)(nb - 1));
+    }
+    if (cost < best - 1e-9) {
+      best = cost;
+      bs = cand[ci];
+    }
   }
   return bs;
 }

@@ -435,39 +435,81 @@ __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
 // ------------------------------------------------------------------------
 // binary32 form (DTWAX_F32), NSTR strands (1: single sequence, 2: fwd/rev)
 // ------------------------------------------------------------------------
-template <int S, bool M3>
-__device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)[S], float s,
-                                          float tin, float dg, const float (&Do)[S],
-                                          float (&Dn)[S]) {
-  float tp = tin, dd = dg;
+// One step of NSTR strands: Do = column c-1 values, Dn = column c values.
+// The strands' chains (min -> add -> min ...) are independent, so they are
+// interleaved cell by cell, and each strand's row costs t*t are computed two
+// cells ahead of the chain; scheduling barriers keep that order (left to
+// itself the scheduler emits one strand's chain as a run of dependent ops).
+// Binary32 VALU ops issue at twice the rate of packed binary16 ones, so the
+// dependent pairs need independent work between them to keep the issue rate.
+template <int S, int NSTR, bool M3>
+__device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)[S],
+                                          const float (&s)[NSTR], const float (&tin)[NSTR],
+                                          const float (&dg)[NSTR], const float (&Do)[NSTR][S],
+                                          float (&Dn)[NSTR][S]) {
+  constexpr int LA = 2;  // cells of look-ahead for the row costs
+  float tp[NSTR], dd[NSTR], c[NSTR][S];
+#pragma unroll
+  for (int st = 0; st < NSTR; st++) {
+    tp[st] = tin[st];
+    dd[st] = dg[st];
+#pragma unroll
+    for (int i = 0; i < LA && i < S; i++) {
+      const float t = __builtin_fmaf(s[st], A[i], Qn[i]);  // == rn(s - q) for a real row
+      c[st][i] = t * t;
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int i = 0; i < S; i++) {
-    float t = __builtin_fmaf(s, A[i], Qn[i]);  // == rn(s - q) for a real row
-    float c = t * t;
-    if constexpr (M3) dd = __builtin_fminf(dd, Do[i]);  // left move
-    float m = __builtin_fminf(tp, dd);
-    float v = c + m;
-    Dn[i] = v;
-    dd = Do[i];
-    tp = v;
+    float m[NSTR];
+#pragma unroll
+    for (int st = 0; st < NSTR; st++) {
+      if constexpr (M3) m[st] = __builtin_fminf(tp[st], __builtin_fminf(dd[st], Do[st][i]));
+      else m[st] = __builtin_fminf(tp[st], dd[st]);
+    }
+    float t[NSTR];
+#pragma unroll
+    for (int st = 0; st < NSTR; st++)
+      if (i + LA < S) t[st] = __builtin_fmaf(s[st], A[i + LA], Qn[i + LA]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int st = 0; st < NSTR; st++) {
+      const float v = c[st][i] + m[st];
+      Dn[st][i] = v;
+      tp[st] = v;
+      dd[st] = Do[st][i];
+    }
+#pragma unroll
+    for (int st = 0; st < NSTR; st++)
+      if (i + LA < S) c[st][i + LA] = t[st] * t[st];
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
+// One column step; the step's column (s_cur) and the next step's top input
+// (tf_cur) were read from LDS a step earlier, and this step reads the next
+// ones (slot base + 65 at most: inside the mirrored ring).
 template <int S, int NSTR, bool M3>
-__device__ __forceinline__ void f32_half_step(const float (&A)[S], const float (&Qn)[S],
-                                              const float* rs, const float* rt, int u,
-                                              float (&tin)[NSTR], float (&dg)[NSTR],
-                                              float (&hist)[NSTR], float (&Do)[NSTR][S],
-                                              float (&Dn)[NSTR][S]) {
+__device__ __forceinline__ void f32_step(const float (&A)[S], const float (&Qn)[S], const float* rs,
+                                         const float* rt, int u, float (&s_cur)[NSTR],
+                                         float (&tf_cur)[NSTR], float (&tin)[NSTR],
+                                         float (&dg)[NSTR], float (&hist)[NSTR],
+                                         const float (&Do)[NSTR][S], float (&Dn)[NSTR][S]) {
+  float s_next[NSTR], tf_next[NSTR];
 #pragma unroll
   for (int st = 0; st < NSTR; st++) {
-    const float s = rs[u * NSTR + st];
-    f32_cells<S, M3>(A, Qn, s, tin[st], dg[st], Do[st], Dn[st]);
-    const float tf = rt[(u + 1) * NSTR + st];
+    s_next[st] = rs[(u + 1) * NSTR + st];
+    tf_next[st] = rt[(u + 2) * NSTR + st];
+  }
+  f32_cells<S, NSTR, M3>(A, Qn, s_cur, tin, dg, Do, Dn);
+#pragma unroll
+  for (int st = 0; st < NSTR; st++) {
     dg[st] = tin[st];
-    float ti = dpp_shr1(tf, Dn[st][S - 1]);
-    tin[st] = ti;
+    tin[st] = dpp_shr1(tf_cur[st], Dn[st][S - 1]);
     hist[st] = dpp_shl1(Dn[st][S - 1], hist[st]);
+    s_cur[st] = s_next[st];
+    tf_cur[st] = tf_next[st];
   }
 }
 
@@ -541,6 +583,13 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
       if (lane == 0 && b == 0 && trow_ext) dg[st] = ring_t[63 * NSTR + st];
     }
 
+    // column of step 0 and the top input of step 1 (then read a step ahead)
+    float s_cur[NSTR], tf_cur[NSTR];
+#pragma unroll
+    for (int st = 0; st < NSTR; st++) {
+      s_cur[st] = ring_s[((64 - lane) & (RING - 1)) * NSTR + st];
+      tf_cur[st] = ring_t[(((64 - lane) & (RING - 1)) + 1) * NSTR + st];
+    }
     for (int32_t T = 0; T < nsteps; T += 64) {
       float pf_s[NSTR], pf_t[NSTR];
 #pragma unroll
@@ -555,8 +604,8 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
         const float* rt = ring_t + (base + g) * NSTR;
 #pragma unroll
         for (int u = 0; u < 8; u += 2) {
-          f32_half_step<S, NSTR, M3>(A, Qn, rs, rt, u, tin, dg, hist, D0, D1);
-          f32_half_step<S, NSTR, M3>(A, Qn, rs, rt, u + 1, tin, dg, hist, D1, D0);
+          f32_step<S, NSTR, M3>(A, Qn, rs, rt, u, s_cur, tf_cur, tin, dg, hist, D0, D1);
+          f32_step<S, NSTR, M3>(A, Qn, rs, rt, u + 1, s_cur, tf_cur, tin, dg, hist, D1, D0);
         }
       }
       const int32_t col = x + T + lane - 63;
