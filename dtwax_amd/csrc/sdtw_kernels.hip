@@ -365,7 +365,9 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     for (int i = 0; i < S; i++) {
       const int32_t r = r0 + i;
       if (r >= 0) {
-        half_t q = (half_t)qp[r];
+        // queries are read once: a non-temporal load keeps them from evicting
+        // the reference, which every work item re-reads, from L2
+        half_t q = (half_t)__builtin_nontemporal_load(&qp[r]);
         B.Qn[i] = hsplat(-q);
         B.A[i] = hsplat((half_t)1.0f);
       } else {
@@ -543,7 +545,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
 #pragma unroll
     for (int i = 0; i < S; i++) {
       const int32_t r = r0 + i;
-      Qn[i] = (r >= 0) ? -qp[r] : 0.0f;
+      Qn[i] = (r >= 0) ? -__builtin_nontemporal_load(&qp[r]) : 0.0f;
       A[i] = (r >= 0) ? 1.0f : 0.0f;
     }
     float* scr = scratch - (int64_t)it.x * NSTR;  // absolute-column view of the band scratch
