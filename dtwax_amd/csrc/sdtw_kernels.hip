@@ -326,6 +326,166 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
   }
 }
 
+// ---- binary16, two columns per step ----------------------------------------
+// Lane L at step t computes columns cA = x + 2t - 2L and cB = cA + 1 of its
+// S rows (a two-column lane skew).  The A column depends on the previous
+// step's B column (its diagonal) and the B column on the A column one row
+// up, so B is computed one row behind A: the pair (A_k, B_(k-1)) are two
+// independent chains, and every dependent op of one has three ops of the
+// other chain or of the look-ahead row costs between it and its producer.
+// This removes the fused form's wait states (its chain min -> fma has no
+// third op per cell to put between links): 21.5 -> 23.1 T cells/s at S = 32
+// on the 48.5 kb x 4096 x 2000 batch, 20.9 -> 22.4 at S = 16; the per-op form
+// runs at S = 32 as before (17.9) and 4.5% faster at S = 16.  At S = 8 the
+// doubled per-step overhead (four lane moves per 16 cells, a 127-step ramp
+// per band) outweighs it (15.9 -> 14.4), so S = 8 keeps the one-column
+// sweep, as do the renormalised form and the three-move recurrence.
+template <int S, int MODE, bool M3>
+constexpr bool use_band2() {
+  return S >= 16 && !M3 && (MODE == DTWAX_F16 || MODE == DTWAX_F16_FUSED);
+}
+constexpr int RING2 = 512;    // slots; column offset o = c - x lives in slot (o + 128) & 511
+constexpr int MIRROR2 = 136;  // slots [0, MIRROR2) are also kept at RING2 + slot
+constexpr int RING2_ALLOC = RING2 + MIRROR2;
+
+template <int S, int MODE>
+__device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 sB, h2 tinA, h2 tinB,
+                                           h2 diagA, h2 (&Bc)[S], h2& lastA) {
+  constexpr bool FUSED = MODE == DTWAX_F16_FUSED;
+  // cA/cB: the pair's row costs t*t (per-op form) or t (fused form)
+  const h2 t0 = hfma2(sA, B.A[0], B.Qn[0]);  // == rn(s - q) for a real row
+  h2 cA = FUSED ? t0 : t0 * t0;
+  h2 cB = hsplat((half_t)0.0f);
+  h2 a1 = tinA, a2 = tinA, b2 = tinB;  // A_(k-1), A_(k-2), B_(k-2)
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k <= S; k++) {
+    h2 mA, mB, nA, nB;
+    if (k < S) mA = hmin2(a1, k == 0 ? diagA : Bc[k - 1]);  // A_k: top A_(k-1), diagonal Bprev_(k-1)
+    if (k >= 1) mB = hmin2(b2, a2);                          // B_(k-1): top B_(k-2), diagonal A_(k-2)
+    if (k + 1 < S) nA = hfma2(sA, B.A[k + 1], B.Qn[k + 1]);
+    if (k < S) nB = hfma2(sB, B.A[k], B.Qn[k]);
+    __builtin_amdgcn_sched_barrier(0);
+    h2 vA;
+    if (k < S) vA = FUSED ? hfma2(cA, cA, mA) : cA + mA;
+    if (k >= 1) {
+      const h2 vB = FUSED ? hfma2(cB, cB, mB) : cB + mB;
+      Bc[k - 1] = vB;  // after A_k read the previous column's row k-1
+      b2 = vB;
+    }
+    if (k + 1 < S) cA = FUSED ? nA : nA * nA;
+    if (k < S) cB = FUSED ? nB : nB * nB;
+    if (k < S) {
+      a2 = a1;
+      a1 = vA;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  lastA = a1;
+}
+
+// One band over columns [x, c1) (+ ramp), two columns per step; band-final
+// min/argmin per strand over [c0, c1) into best/posf/posr, the band's last
+// row to io.orow.
+template <int S, int MODE>
+__device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const SweepIO& io, h2* ring_s,
+                                          h2* ring_t, int lane, h2& best, int32_t& posf, int32_t& posr) {
+  const h2 INF = hsplat((half_t)__builtin_inff());
+  const h2 ZERO = hsplat((half_t)0.0f);
+  const int32_t x = io.x, c0 = io.c0, c1 = io.c1;
+  const h2* __restrict__ ref = io.ref;
+  // top rows from trow, or 0 (free start): loads without branches
+  const h2* tsrc = io.trow ? io.trow : ref;
+  const h2 tmask = io.trow ? hsplat((half_t)1.0f) : ZERO;
+  auto put = [&](h2* ring, int slot, h2 v) {
+    ring[slot] = v;
+    if (slot < MIRROR2) ring[RING2 + slot] = v;
+  };
+  // Prologue: offsets [-128, 0) are before the item (their cells are +inf
+  // whatever the column holds): zeros; offsets [0, 130) from memory.
+  put(ring_s, lane, ZERO);
+  put(ring_s, 64 + lane, ZERO);
+  put(ring_t, lane, ZERO);
+  put(ring_t, 64 + lane, ZERO);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int o = 64 * k + lane;
+    if (o < 130) {
+      put(ring_s, o + 128, ref[x + o]);
+      put(ring_t, o + 128, tsrc[x + o] * tmask);
+    }
+  }
+  // a resumed row also gives lane 0 its first diagonal, trow[x - 1] (slot 127)
+  if (io.top_diag && lane == 0) put(ring_t, 127, io.trow[x - 1]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+
+  h2 Bc[S];
+#pragma unroll
+  for (int i = 0; i < S; i++) Bc[i] = INF;
+  h2 tinA = INF, tinB = INF, diagA = INF, histA = INF, histB = INF;
+  if (lane == 0) {
+    tinA = ring_t[128];
+    tinB = ring_t[129];
+    if (io.top_diag) diagA = ring_t[127];
+  }
+  const int base0 = (128 - 2 * lane) & (RING2 - 1);
+  h2 sA = ring_s[base0], sB = ring_s[base0 + 1];
+
+  const int32_t nsteps = (((c1 - x + 127) >> 1) + 63) & ~63;
+  for (int32_t T = 0; T < nsteps; T += 64) {
+    // offsets [2T + 130, 2T + 258): written to the ring after this block
+    h2 pf_s[2], pf_t[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int32_t c = x + 2 * T + 130 + 64 * h + lane;
+      pf_s[h] = ref[c];
+      pf_t[h] = tsrc[c] * tmask;
+    }
+    const int base = (2 * T + 128 - 2 * lane) & (RING2 - 1);
+#pragma unroll 1
+    for (int g = 0; g < 64; g += 4) {
+      const h2* rs = ring_s + base + 2 * g;
+      const h2* rt = ring_t + base + 2 * g;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        // next step's columns; the top inputs of the next step (lane 0)
+        const h2 nA = rs[2 * u + 2], nB = rs[2 * u + 3];
+        const h2 tfA = rt[2 * u + 2], tfB = rt[2 * u + 3];
+        h2 lastA;
+        f16_cells2<S, MODE>(B, sA, sB, tinA, tinB, diagA, Bc, lastA);
+        diagA = tinB;
+        tinA = dpp_shr1(tfA, lastA);
+        tinB = dpp_shr1(tfB, Bc[S - 1]);
+        histA = dpp_shl1(lastA, histA);
+        histB = dpp_shl1(Bc[S - 1], histB);
+        sA = nA;
+        sB = nB;
+      }
+    }
+    // lane j holds the last row at columns x + 2(T + j) - 126 (A) and + 1 (B)
+    const int32_t colA = x + 2 * (T + lane) - 126;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int32_t col = colA + h;
+      const h2 v = h ? histB : histA;
+      if (io.final_band && col >= c0 && col < c1) {
+        if (v.x < best.x) { best.x = v.x; posf = col; }
+        if (v.y < best.y) { best.y = v.y; posr = col; }
+      }
+      if (io.orow && col >= x && col < c1 && (!io.final_band || col >= c0)) io.orow[col] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int slot = (2 * T + 258 + 64 * h + lane) & (RING2 - 1);
+      put(ring_s, slot, pf_s[h]);
+      put(ring_t, slot, pf_t[h]);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 __device__ __forceinline__ half_t clamp_half(float v) { return (half_t)v; }
 
 template <int S, int MODE, bool M3>
@@ -389,7 +549,10 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     h2 rowmin = hsplat((half_t)__builtin_inff());
     h2 Kl = Kb;
     if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = Kb + prev_min;
-    f16_sweep<S, MODE, M3>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
+    if constexpr (use_band2<S, MODE, M3>())
+      f16_band2<S, MODE>(B, io, ring_s, ring_t, lane, best, posf, posr);
+    else
+      f16_sweep<S, MODE, M3>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
     if (MODE == DTWAX_F16_RENORM) {
       // lanes whose subtraction lies on the path to the last row
       const int32_t first = 0;
@@ -422,8 +585,9 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
 
 template <int S, int MODE, bool M3>
 __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
-  __shared__ h2 ring_s[RING_ALLOC];
-  __shared__ h2 ring_t[RING_ALLOC];
+  constexpr int RA = use_band2<S, MODE, M3>() ? RING2_ALLOC : RING_ALLOC;
+  __shared__ h2 ring_s[RA];
+  __shared__ h2 ring_t[RA];
   const int lane = threadIdx.x;
   h2* scratch = (h2*)p.scratch + (int64_t)blockIdx.x * p.scratch_stride + SCRATCH_PADL;
   for (;;) {
