@@ -216,7 +216,9 @@ static int64_t scratch_budget() {
   return std::max<int64_t>((int64_t)1 << 30, std::min<int64_t>((int64_t)32 << 30, (int64_t)(free_b / 4)));
 }
 
-static bool valid_segment(int S) { return S == 8 || S == 16 || S == 25 || S == 32; }
+static bool valid_segment(int S) {
+  return S == 8 || S == 12 || S == 16 || S == 20 || S == 25 || S == 28 || S == 32;
+}
 
 static int auto_segment(const std::vector<int32_t>& lens) {
   // binary32: one width per batch, the least padded rows x the measured time
@@ -258,7 +260,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   dtwax_score_opts o{};
   if (opts) o = *opts;
   if (o.precision < DTWAX_F16 || o.precision > DTWAX_F32) return fail(DTWAX_EINVAL, "bad precision");
-  if (o.segment && !valid_segment(o.segment)) return fail(DTWAX_EINVAL, "segment must be 8, 16, 25 or 32");
+  if (o.segment && !valid_segment(o.segment)) return fail(DTWAX_EINVAL, "segment must be 8, 12, 16, 20, 25, 28 or 32");
   if (o.ref_chunk < 0) return fail(DTWAX_EINVAL, "ref_chunk < 0");
   if (o.waves < 0) return fail(DTWAX_EINVAL, "waves < 0");
   if (o.compat_quirks) return fail(DTWAX_EINVAL, "compat_quirks is only available through dtwax_distances");
@@ -293,9 +295,10 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   // subtracts each band's exact row minimum, so it defaults to the shortest
   // bands (64 x 8 rows); binary32 takes one width for the batch; the
   // binary16 forms pick, per read, the width with the least padded work:
-  // padded rows x the measured time per computed cell of that width
-  // (relative to S = 32: 1.072 at 8, 1.021 at 16, 1.026 at 25) x 1.5% per
-  // extra band (band hand-over through scratch).
+  // padded rows x the time per computed cell of that width relative to
+  // S = 32 (measured: 1.072 at 8, 1.021 at 16, 1.026 at 25; 12, 20 and 28
+  // interpolated: 1.045, 1.02, 1.01) x 1.5% per extra band (band hand-over
+  // through scratch).
   std::vector<int> seg_of((size_t)n_reads, 0);
   if (o.segment) {
     std::fill(seg_of.begin(), seg_of.end(), o.segment);
@@ -304,8 +307,8 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   } else if (o.precision == DTWAX_F32) {
     std::fill(seg_of.begin(), seg_of.end(), auto_segment(lens));
   } else {
-    static const int cand[4] = {8, 16, 25, 32};
-    static const double f0[4] = {1.072, 1.021, 1.026, 1.0};
+    static const int cand[7] = {8, 12, 16, 20, 25, 28, 32};
+    static const double f0[7] = {1.072, 1.045, 1.021, 1.02, 1.026, 1.01, 1.0};
     auto cost = [&](int64_t q, int ci) {
       const int64_t BR = 64 * cand[ci];
       const int64_t nb = (q + BR - 1) / BR;
@@ -313,7 +316,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     };
     for (int32_t r = 0; r < n_reads; r++) {
       int best = 0;
-      for (int ci = 1; ci < 4; ci++)
+      for (int ci = 1; ci < 7; ci++)
         if (cost(lens[r], ci) < cost(lens[r], best) - 1e-9) best = ci;
       seg_of[(size_t)r] = cand[best];
     }

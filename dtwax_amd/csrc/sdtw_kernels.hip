@@ -897,8 +897,11 @@ template <int MODE>
 static hipError_t launch_f16_s(const KParams& p, int S, int grid, bool m3, hipStream_t st) {
   switch (S) {
     case 8: return launch_f16<8, MODE>(p, grid, m3, st);
+    case 12: return launch_f16<12, MODE>(p, grid, m3, st);
     case 16: return launch_f16<16, MODE>(p, grid, m3, st);
+    case 20: return launch_f16<20, MODE>(p, grid, m3, st);
     case 25: return launch_f16<25, MODE>(p, grid, m3, st);
+    case 28: return launch_f16<28, MODE>(p, grid, m3, st);
     case 32: return launch_f16<32, MODE>(p, grid, m3, st);
   }
   return hipErrorInvalidValue;
@@ -913,8 +916,11 @@ hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int gri
     case DTWAX_F32:
       switch (S) {
         case 8: return launch_f32<8>(p, grid, nstr, m3, st);
+        case 12: return launch_f32<12>(p, grid, nstr, m3, st);
         case 16: return launch_f32<16>(p, grid, nstr, m3, st);
+        case 20: return launch_f32<20>(p, grid, nstr, m3, st);
         case 25: return launch_f32<25>(p, grid, nstr, m3, st);
+        case 28: return launch_f32<28>(p, grid, nstr, m3, st);
         case 32: return launch_f32<32>(p, grid, nstr, m3, st);
       }
   }
@@ -960,8 +966,11 @@ int occupancy_f16(int precision, int S, bool m3) {
 #define OCC_S(MODE)            \
   switch (S) {                 \
     case 8: OCC(MODE, 8) break;   \
+    case 12: OCC(MODE, 12) break; \
     case 16: OCC(MODE, 16) break; \
+    case 20: OCC(MODE, 20) break; \
     case 25: OCC(MODE, 25) break; \
+    case 28: OCC(MODE, 28) break; \
     case 32: OCC(MODE, 32) break; \
   }
   switch (precision) {
@@ -981,7 +990,10 @@ int occupancy_f32(int S, int nstr, bool m3) {
   else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<SS, NS, false>, 64, 0);
   switch (S) {
     case 8: if (nstr == 1) { OCC(8, 1) } else { OCC(8, 2) } break;
+    case 12: if (nstr == 1) { OCC(12, 1) } else { OCC(12, 2) } break;
     case 16: if (nstr == 1) { OCC(16, 1) } else { OCC(16, 2) } break;
+    case 20: if (nstr == 1) { OCC(20, 1) } else { OCC(20, 2) } break;
+    case 28: if (nstr == 1) { OCC(28, 1) } else { OCC(28, 2) } break;
     case 25: if (nstr == 1) { OCC(25, 1) } else { OCC(25, 2) } break;
     case 32: if (nstr == 1) { OCC(32, 1) } else { OCC(32, 2) } break;
   }

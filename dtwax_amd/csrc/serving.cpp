@@ -92,7 +92,9 @@ struct Lane {
 constexpr uint32_t kHalf2Inf = 0x7c007c00u;
 constexpr uint32_t kFloatInf = 0x7f800000u;
 
-bool valid_segment(int S) { return S == 8 || S == 16 || S == 25 || S == 32; }
+bool valid_segment(int S) {
+  return S == 8 || S == 12 || S == 16 || S == 20 || S == 25 || S == 28 || S == 32;
+}
 
 }  // namespace
 
@@ -484,13 +486,13 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   if (o.chunk_len % 64) return set_error(DTWAX_EINVAL, "chunk_len must be a multiple of 64");
   const int rows_per_lane = o.chunk_len / 64;
   if (o.segment == 0) {
-    for (int S : {32, 25, 16, 8})
+    for (int S : {32, 28, 25, 20, 16, 12, 8})
       if (rows_per_lane % S == 0) { o.segment = S; break; }
     if (o.segment == 0)
-      return set_error(DTWAX_EINVAL, "chunk_len / 64 must be a multiple of 8, 16, 25 or 32");
+      return set_error(DTWAX_EINVAL, "chunk_len / 64 must be a multiple of 8, 12, 16, 20, 25, 28 or 32");
   }
   if (!valid_segment(o.segment) || rows_per_lane % o.segment)
-    return set_error(DTWAX_EINVAL, "segment must be 8, 16, 25 or 32 and divide chunk_len / 64");
+    return set_error(DTWAX_EINVAL, "segment must be 8, 12, 16, 20, 25, 28 or 32 and divide chunk_len / 64");
   if (o.min_ref_chunk < 0 || o.hbm_budget < 0) return set_error(DTWAX_EINVAL, "negative size");
   if (o.min_ref_chunk == 0) o.min_ref_chunk = 4 * o.chunk_len;
   if (!(o.threshold > 0.0f)) o.threshold = INFINITY;  // no early non-target decision

@@ -72,7 +72,7 @@ typedef struct {
 
 typedef struct {
   int32_t precision;     /* dtwax_precision */
-  int32_t segment;       /* query rows per lane (8, 16, 25 or 32) for every read; 0 = automatic:
+  int32_t segment;       /* query rows per lane (8, 12, 16, 20, 25, 28 or 32) for every read; 0 = automatic:
                             per read for DTWAX_F16 and DTWAX_F16_FUSED (the width with the
                             least padded work, one persistent launch per width used), 8 for
                             DTWAX_F16_RENORM, one width per batch for DTWAX_F32 */
@@ -207,7 +207,7 @@ int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
  * 512) spread over the sorted lengths, against the first sample_cols
  * reference columns (<= 0: 2^20), replicated until every candidate has at
  * least 6 work items per resident wave (steady state, as in the full run).
- * It times segment in {0 = per-read widths, 8, 16, 25, 32} ({8, 16} for
+ * It times segment in {0 = per-read widths, 8, 12, 16, 20, 25, 28, 32} ({8, 16} for
  * DTWAX_F16_RENORM) at ref_chunk min(65536, sample_cols); the fastest per
  * counted cell wins.  ref_chunk is then the plan's own rule for the whole
  * workload at that width (whole strands, or the longest chunks whose band
@@ -280,7 +280,7 @@ typedef struct {
   int32_t chunk_len;     /* samples per chunk, a multiple of 64 (0: 1600) */
   int32_t max_chunks;    /* chunks before a TARGET decision (0: 4) */
   int32_t precision;     /* DTWAX_F16 or DTWAX_F32 */
-  int32_t segment;       /* rows per lane, dividing chunk_len / 64 (0: largest of 32, 25, 16, 8) */
+  int32_t segment;       /* rows per lane, dividing chunk_len / 64 (0: largest of 32, 28, 25, 20, 16, 12, 8) */
   int32_t norm;          /* dtwax_norm_mode, statistics of the read's first chunk */
   double eps;            /* DTWAX_NORM_MEANSTD epsilon */
   float threshold;       /* on min(fwd, rev); <= 0: no early NONTARGET */

@@ -28,7 +28,7 @@ def _oracle3_f16(reads, ref):
     return O.f16_to_f32(sc.ravel()).reshape(sc.shape), am
 
 
-@pytest.mark.parametrize("segment", [0, 8, 16, 25, 32])
+@pytest.mark.parametrize("segment", [0, 8, 12, 16, 20, 25, 28, 32])
 def test_f16_three_moves_bit_exact(ctx, segment):
     rng = np.random.default_rng(300 + segment)
     ref = rng.standard_normal((2, 3001)).astype(np.float32)
@@ -93,7 +93,7 @@ def test_three_moves_options(ctx):
     flat, offs, lns = D.pack_reads(reads)
     d_q, d_off = D.DeviceArray.from_host(flat), D.DeviceArray.from_host(offs)
     best, table = ctx.autotune(d_q, d_off, lns, sample_reads=16, precision="f16", moves=3)
-    assert best["ref_chunk"] == 0 and len(table) == 5
+    assert best["ref_chunk"] == 0 and len(table) == 8
     sc, am = ctx.score(reads, precision="f16", moves=3, **best)
     osc, oam = _oracle3_f16(reads, ref)
     assert np.array_equal(sc, osc) and np.array_equal(am, oam)

@@ -40,7 +40,7 @@ def _oracle_f16(reads, ref, fused=False):
     return O.f16_to_f32(sc.ravel()).reshape(sc.shape), am
 
 
-@pytest.mark.parametrize("segment", [8, 16, 25, 32])
+@pytest.mark.parametrize("segment", [8, 12, 16, 20, 25, 28, 32])
 def test_f16_bit_exact(ctx, segment):
     rng = np.random.default_rng(segment)
     ref = rng.standard_normal((2, 3001)).astype(np.float32)
@@ -52,12 +52,13 @@ def test_f16_bit_exact(ctx, segment):
     assert np.array_equal(am, oam)
 
 
-def test_f16_fused_bit_exact(ctx):
+@pytest.mark.parametrize("segment", [8, 16, 20, 32])
+def test_f16_fused_bit_exact(ctx, segment):
     rng = np.random.default_rng(41)
     ref = rng.standard_normal((2, 900)).astype(np.float32)
     ctx.upload_reference(ref)
-    reads = [rng.standard_normal(n).astype(np.float32) for n in (1, 64, 130, 700)]
-    sc, am = ctx.score(reads, precision="f16_fused", segment=8)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in (1, 64, 130, 700, 2100)]
+    sc, am = ctx.score(reads, precision="f16_fused", segment=segment)
     osc, oam = _oracle_f16(reads, ref, fused=True)
     assert np.array_equal(sc, osc) and np.array_equal(am, oam)
 
@@ -68,7 +69,7 @@ def test_f32_bit_exact(ctx, nstr):
     ref = rng.standard_normal((nstr, 2500)).astype(np.float32)
     ctx.upload_reference(ref)
     reads = [rng.standard_normal(n).astype(np.float32) for n in LENS[:-2]]
-    for seg in (8, 16):
+    for seg in (8, 12, 16, 20, 28):
         sc, am = ctx.score(reads, precision="f32", segment=seg)
         osc, oam = O.batch_f32(reads, ref)
         assert np.array_equal(sc[:, :nstr], osc)
@@ -256,8 +257,8 @@ def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     d_q, d_off = D.DeviceArray.from_host(flat), D.DeviceArray.from_host(offs)
     cache = tmp_path / "tune.txt"
     best, table = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, cache_path=cache)
-    assert len(table) == 5  # widths 0 (per read), 8, 16, 25, 32
-    assert best["segment"] in (0, 8, 16, 25, 32) and best["ref_chunk"] > 0
+    assert len(table) == 8  # widths 0 (per read), 8, 12, 16, 20, 25, 28, 32
+    assert best["segment"] in (0, 8, 12, 16, 20, 25, 28, 32) and best["ref_chunk"] > 0
     top = table[np.argmax(table["gcups"])]
     assert top["segment"] == best["segment"]
     # the chunk is the plan's own rule for the whole workload at that width
@@ -276,7 +277,7 @@ def test_plan_and_score_argument_errors(ctx):
     """Bad options fail with a status and a message, not a fault."""
     rng = np.random.default_rng(4)
     ctx.upload_reference(rng.standard_normal((2, 1000)).astype(np.float32))
-    for kw in (dict(segment=12), dict(ref_chunk=-1), dict(precision=7)):
+    for kw in (dict(segment=13), dict(ref_chunk=-1), dict(precision=7)):
         with pytest.raises(D.DtwaxError):
             ctx.plan(np.array([100, 200], np.int32), **kw)
     with pytest.raises(D.DtwaxError, match="at least one sample"):
