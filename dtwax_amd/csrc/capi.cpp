@@ -296,9 +296,9 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   // bands (64 x 8 rows); binary32 takes one width for the batch; the
   // binary16 forms pick, per read, the width with the least padded work:
   // padded rows x the time per computed cell of that width relative to
-  // S = 32 (measured: 1.072 at 8, 1.021 at 16, 1.026 at 25; 12, 20 and 28
-  // interpolated: 1.045, 1.02, 1.01) x 1.5% per extra band (band hand-over
-  // through scratch).
+  // S = 32 x 1.5% per extra band (band hand-over through scratch).  The
+  // width factors are measured on 4096 x 2000-sample batches with the band
+  // factor taken out: 1.03 at S = 8, 1.011 at 12 and 16, 1.0 from 20 up.
   std::vector<int> seg_of((size_t)n_reads, 0);
   if (o.segment) {
     std::fill(seg_of.begin(), seg_of.end(), o.segment);
@@ -308,7 +308,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     std::fill(seg_of.begin(), seg_of.end(), auto_segment(lens));
   } else {
     static const int cand[7] = {8, 12, 16, 20, 25, 28, 32};
-    static const double f0[7] = {1.072, 1.045, 1.021, 1.02, 1.026, 1.01, 1.0};
+    static const double f0[7] = {1.03, 1.011, 1.011, 1.0, 1.0, 1.0, 1.0};
     auto cost = [&](int64_t q, int ci) {
       const int64_t BR = 64 * cand[ci];
       const int64_t nb = (q + BR - 1) / BR;
