@@ -222,14 +222,15 @@ static bool valid_segment(int S) {
 
 static int auto_segment(const std::vector<int32_t>& lens) {
   // binary32: one width per batch, the least padded rows x the measured time
-  // per computed cell of that width relative to S = 16 (1.147 at 8, 1.058 at
-  // 25, 1.088 at 32: S = 32 holds 2 waves per SIMD, S = 16 three) x 1.5% per
-  // extra band
-  static const int cand[] = {8, 16, 25, 32};
-  static const double f0[] = {1.147, 1.0, 1.058, 1.088};
+  // per computed cell of that width relative to S = 16 (4096 x 2000-sample
+  // batches, band factor taken out: 1.066 at 8, 1.062 at 12, 1.102 at 20,
+  // 1.084 at 25, 1.076 at 28, 1.109 at 32; from S = 20 up 2 waves fit a SIMD,
+  // at S = 16 three) x 1.5% per extra band
+  static const int cand[] = {8, 12, 16, 20, 25, 28, 32};
+  static const double f0[] = {1.066, 1.062, 1.0, 1.102, 1.084, 1.076, 1.109};
   double best = 1e300;
   int bs = 16;
-  for (int ci = 0; ci < 4; ci++) {
+  for (int ci = 0; ci < 7; ci++) {
     const int64_t BR = 64 * cand[ci];
     double cost = 0;
     for (int32_t q : lens) {
