@@ -229,9 +229,11 @@ def test_mixed_segment_plan_exact(ctx):
     rng = np.random.default_rng(91)
     ref = rng.standard_normal((2, 4001)).astype(np.float32)
     ctx.upload_reference(ref)
-    # lengths whose least-padding widths are 8, 16, 25 and 32; 64-column
-    # pieces give every width group enough work items to keep its own launch
-    lens = [500] * 240 + [1000] * 240 + [1600] * 240 + [2048] * 240 + [777, 3001]
+    # lengths whose least-padding widths are 8, 12, 16, 20, 25, 28 and 32;
+    # 64-column pieces give every width group enough work items to keep its
+    # own launch
+    lens = ([500] * 140 + [760] * 140 + [1000] * 140 + [1270] * 140 + [1600] * 140 +
+            [1790] * 140 + [2048] * 140 + [777, 3001])
     reads = [rng.standard_normal(n).astype(np.float32) for n in lens]
     flat, offs, lns = D.pack_reads(reads)
     p = ctx.plan(lns, precision="f16", segment=0, ref_chunk=64)
