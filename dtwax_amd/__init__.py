@@ -92,6 +92,11 @@ def lib():
             "dtwax_ru_poll": ([vp, vp, i32, C.POINTER(i32), i32], C.c_int),
             "dtwax_ru_stop": ([vp], C.c_int),
             "dtwax_vbz_decode": ([vp, i64, vp, i32, vp, i64, C.POINTER(i64)], C.c_int),
+            "dtwax_pod5_open": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+            "dtwax_pod5_close": ([vp], C.c_int),
+            "dtwax_pod5_count": ([vp, C.POINTER(i64), C.POINTER(i64)], C.c_int),
+            "dtwax_pod5_reads": ([vp, i64, i64, vp], C.c_int),
+            "dtwax_pod5_signal": ([vp, vp, i64, i64, i64, vp, i64, vp, i32], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -113,7 +118,8 @@ def exported_symbols():
         "dtwax_plan_set_timing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
         "dtwax_autotune", "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
         "dtwax_ru_release", "dtwax_ru_start", "dtwax_ru_submit", "dtwax_ru_poll", "dtwax_ru_stop",
-        "dtwax_vbz_decode",
+        "dtwax_vbz_decode", "dtwax_pod5_open", "dtwax_pod5_close", "dtwax_pod5_count",
+        "dtwax_pod5_reads", "dtwax_pod5_signal",
     ]
 
 
@@ -323,6 +329,74 @@ def vbz_decode(chunk, n_max, cd_values=(0, 2, 1, 1)):
     _check(lib().dtwax_vbz_decode(buf.ctypes.data, buf.size, cd.ctypes.data, cd.size,
                                   out.ctypes.data, n_max, C.byref(n)))
     return out[: n.value].copy()
+
+
+POD5_READ_DTYPE = np.dtype([("n_samples", np.int64), ("start_sample", np.int64),
+                            ("channel", np.int32), ("read_number", np.int32), ("n_rows", np.int32),
+                            ("well", np.int32), ("calibration_offset", np.float32),
+                            ("calibration_scale", np.float32), ("read_id", "S40")])
+assert POD5_READ_DTYPE.itemsize == 80
+
+
+class Pod5File:
+    """A POD5 file read by the library's own parser (dtwax_pod5_*): the
+    container footer, the Reads and Signal Arrow tables, and VBZ or plain
+    int16 signal rows.  Host-side; no GPU needed."""
+
+    def __init__(self, path):
+        self._h = C.c_void_p()
+        _check(lib().dtwax_pod5_open(os.fsencode(path), C.byref(self._h)))
+        n, rows = C.c_int64(), C.c_int64()
+        _check(lib().dtwax_pod5_count(self._h, C.byref(n), C.byref(rows)))
+        self.n_reads, self.n_signal_rows = n.value, rows.value
+        self._info = np.zeros(self.n_reads, POD5_READ_DTYPE)
+        if self.n_reads:
+            _check(lib().dtwax_pod5_reads(self._h, 0, self.n_reads, self._info.ctypes.data))
+
+    def close(self):
+        if self._h:
+            lib().dtwax_pod5_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reads(self):
+        """Per-read metadata (POD5_READ_DTYPE), table order."""
+        return self._info.copy()
+
+    def read_ids(self):
+        return [r.decode() for r in self._info["read_id"]]
+
+    def signal(self, reads=None, skip=0, take=None, threads=0):
+        """Samples [skip, skip + take) of the listed reads (all by default).
+        take=None: whole reads, returned as a list of int16 arrays; an int:
+        a (n, take) int16 array and the per-read counts written."""
+        idx = np.arange(self.n_reads, dtype=np.int64) if reads is None else \
+            np.ascontiguousarray(reads, np.int64)
+        if idx.size and (idx.min() < 0 or idx.max() >= self.n_reads):
+            raise DtwaxError("read index outside the file")
+        n_out = np.zeros(max(1, idx.size), np.int64)
+        if take is None:
+            have = np.maximum(0, self._info["n_samples"][idx] - skip) if idx.size else idx
+            dst = np.empty(max(1, int(have.sum())), np.int16)
+            _check(lib().dtwax_pod5_signal(self._h, idx.ctypes.data, idx.size, skip, -1,
+                                           dst.ctypes.data, int(have.sum()), n_out.ctypes.data, threads))
+            ends = np.cumsum(have)
+            return [dst[e - h:e].copy() for e, h in zip(ends, have)]
+        dst = np.zeros((max(1, idx.size), max(1, take)), np.int16)
+        _check(lib().dtwax_pod5_signal(self._h, idx.ctypes.data, idx.size, skip, take,
+                                       dst.ctypes.data, idx.size * take, n_out.ctypes.data, threads))
+        return dst[: idx.size, :take], n_out[: idx.size]
 
 
 def read_fasta(path):

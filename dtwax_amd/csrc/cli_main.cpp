@@ -8,7 +8,9 @@
 //   * reference: the last FASTA entry (earlier entries are printed as
 //     "name : content", load_reference.hpp:164-212), reverse complement,
 //     squiggle over the first REF_LEN k-mers per strand;
-//   * reads: every *.fast5 in readdir order, every read group in name order;
+//   * reads: every *.fast5 in readdir order, every read group in name order
+//     (and every *.pod5 in the same listing, reads in table order, named
+//     "read_<uuid>"; the reference reads FAST5 only);
 //     reads shorter than ADAPTER_LEN + QUERY_LEN are skipped; samples
 //     [ADAPTER_LEN, ADAPTER_LEN + QUERY_LEN) are kept as raw ADC values;
 //   * mean/std normalisation (eps 1e-4), binary16 sDTW of both strands;
@@ -117,6 +119,48 @@ bool parse(int argc, char** argv, Options& o) {
     }
   }
   if (o.query_len < 1 || o.adapter_len < 0) return false;
+  return true;
+}
+
+// Reads of one POD5 file with at least adapter + query samples: samples
+// [adapter, adapter + query) as raw ADC values, named "read_<uuid>" as a
+// FAST5 read group of the same read is.  Reads that cannot be decoded are
+// reported and skipped, as unreadable FAST5 reads are.
+bool load_pod5(const std::string& path, int adapter, int query, std::vector<std::string>& ids,
+               std::vector<float>& raw, int64_t& invalid, int64_t& valid) {
+  dtwax_pod5* f = nullptr;
+  if (dtwax_pod5_open(path.c_str(), &f) != DTWAX_OK) {
+    std::cerr << dtwax_last_error() << std::endl;
+    return false;
+  }
+  int64_t n = 0;
+  dtwax_pod5_count(f, &n, nullptr);
+  std::vector<dtwax_pod5_read_info> info((size_t)n);
+  if (n) dtwax_pod5_reads(f, 0, n, info.data());
+  std::vector<int64_t> sel;
+  for (int64_t i = 0; i < n; i++) {
+    if (info[(size_t)i].n_samples < (int64_t)adapter + query) invalid++;
+    else sel.push_back(i);
+  }
+  std::vector<int16_t> buf(sel.size() * (size_t)query + 1);
+  std::vector<int64_t> got(sel.size() + 1);
+  std::vector<char> ok(sel.size(), 1);
+  if (dtwax_pod5_signal(f, sel.data(), (int64_t)sel.size(), adapter, query, buf.data(), (int64_t)buf.size(),
+                        got.data(), 0) != DTWAX_OK) {
+    for (size_t j = 0; j < sel.size(); j++)  // one by one: skip only the reads that fail
+      if (dtwax_pod5_signal(f, &sel[j], 1, adapter, query, buf.data() + j * (size_t)query, query, &got[j], 1) !=
+          DTWAX_OK) {
+        std::cerr << dtwax_last_error() << std::endl;
+        ok[j] = 0;
+      }
+  }
+  for (size_t j = 0; j < sel.size(); j++) {
+    if (!ok[j]) continue;
+    valid++;
+    ids.push_back(std::string("read_") + info[(size_t)sel[j]].read_id);
+    raw.insert(raw.end(), buf.begin() + (int64_t)(j * (size_t)query), buf.begin() + (int64_t)((j + 1) * (size_t)query));
+  }
+  dtwax_pod5_close(f);
   return true;
 }
 
@@ -377,12 +421,16 @@ int main(int argc, char** argv) {
   Timer t_data;
   std::cout << "cuDTW:: Loading ONT fast5 reads from " << o.dir << "\n";
   bool opened = false;
-  std::vector<std::string> files = dtwax_io::list_fast5(o.dir, &opened);
+  std::vector<std::string> files = dtwax_io::list_fast5(o.dir, &opened, true);
   const int64_t need = (int64_t)o.query_len + o.adapter_len;
   std::vector<std::string> ids;
   std::vector<float> raw;
   int64_t invalid = 0, valid = 0;
   for (const auto& f : files) {
+    if (f.size() > 5 && f.compare(f.size() - 5, 5, ".pod5") == 0) {
+      load_pod5(f, o.adapter_len, o.query_len, ids, raw, invalid, valid);
+      continue;
+    }
     std::vector<dtwax_io::Fast5Read> reads;
     std::string err;
     bool ok = dtwax_io::read_fast5_file(f, reads, err);

@@ -90,7 +90,7 @@ H5Api& h5() {
 }
 }  // namespace
 
-std::vector<std::string> list_fast5(const std::string& dir, bool* opened) {
+std::vector<std::string> list_fast5(const std::string& dir, bool* opened, bool with_pod5) {
   std::vector<std::string> out;
   DIR* d = opendir(dir.c_str());
   if (opened) *opened = d != nullptr;
@@ -101,7 +101,7 @@ std::vector<std::string> list_fast5(const std::string& dir, bool* opened) {
     // (a name with no '.' is compared whole)
     size_t dot = name.rfind('.');
     std::string ext = name.substr(dot == std::string::npos ? 0 : dot + 1);
-    if (ext == "fast5") out.push_back(dir + name);
+    if (ext == "fast5" || (with_pod5 && ext == "pod5")) out.push_back(dir + name);
   }
   closedir(d);
   return out;

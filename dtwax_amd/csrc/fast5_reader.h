@@ -18,10 +18,10 @@ struct Fast5Read {
   ChannelParams channel;
 };
 
-// "*.fast5" entries of `dir` in readdir order, each as dir + name (the folder
-// argument is expected to end in '/', as the reference concatenates it;
-// src/include/generate_load_squiggle.hpp:132-157).
-std::vector<std::string> list_fast5(const std::string& dir, bool* opened);
+// "*.fast5" entries of `dir` (and "*.pod5" ones when with_pod5) in readdir
+// order, each as dir + name (the folder argument is expected to end in '/',
+// as the reference concatenates it; src/include/generate_load_squiggle.hpp:132-157).
+std::vector<std::string> list_fast5(const std::string& dir, bool* opened, bool with_pod5 = false);
 
 // Read groups of one multi-read file in HDF5 name order
 // (fast5/nanopolish_fast5_io.cpp:72-112).  Returns false if the file cannot

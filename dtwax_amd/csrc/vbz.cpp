@@ -99,6 +99,55 @@ int64_t data_bytes(const uint8_t* blk, int64_t size, int64_t n, const Layout& L)
 
 }  // namespace
 
+namespace dtwax {
+
+// One zstd frame whose content size is announced and at most max_out bytes.
+bool zstd_frame(const uint8_t* src, size_t n, size_t max_out, std::vector<uint8_t>& out, std::string& err) {
+  Zstd& z = zstd();
+  if (!z.ok) {
+    err = z.err;
+    return false;
+  }
+  const unsigned long long fs = z.frame_size(src, n);
+  if (fs > max_out) {  // also the unknown / error sentinels (the largest values)
+    err = "not a zstd frame of the expected size";
+    return false;
+  }
+  out.resize((size_t)fs);
+  const size_t got = z.decompress(out.data(), out.size(), src, n);
+  if (z.is_error(got)) {
+    err = std::string("zstd: ") + z.error_name(got);
+    return false;
+  }
+  if (got != fs) {
+    err = "zstd frame shorter than announced";
+    return false;
+  }
+  return true;
+}
+
+// 1-bit-key stream-VByte block of n zig-zag 16-bit differences (the first
+// against 0), which must account for every byte of the block.
+bool svb16_decode(const uint8_t* blk, size_t size, int64_t n, int16_t* dst) {
+  if (data_bytes(blk, (int64_t)size, n, kLayouts[2]) < 0) return false;
+  const uint8_t* p = blk + (n + 7) / 8;
+  uint16_t prev = 0;
+  for (int64_t i = 0; i < n; i++) {
+    uint16_t u = p[0];
+    if ((blk[i >> 3] >> (i & 7)) & 1) {
+      u = (uint16_t)(u | (uint16_t)p[1] << 8);
+      p += 2;
+    } else {
+      p += 1;
+    }
+    prev = (uint16_t)(prev + (uint16_t)((u >> 1) ^ (uint16_t)(0u - (u & 1u))));
+    dst[i] = (int16_t)prev;
+  }
+  return true;
+}
+
+}  // namespace dtwax
+
 extern "C" int dtwax_vbz_decode(const void* src, int64_t src_size, const uint32_t* cd_values, int32_t n_cd_values,
                                 int16_t* dst, int64_t dst_cap, int64_t* n_out) {
   using dtwax::set_error;

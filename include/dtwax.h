@@ -158,6 +158,45 @@ int dtwax_squiggle_build(const char* seq, int64_t seq_len, int32_t k,
 int dtwax_vbz_decode(const void* src, int64_t src_size, const uint32_t* cd_values,
                      int32_t n_cd_values, int16_t* dst, int64_t dst_cap, int64_t* n_out);
 
+/* ---- POD5 input ---------------------------------------------------------
+ * No counterpart in the reference, which reads FAST5 only
+ * (fast5/nanopolish_fast5_io.cpp:72-194): ONT's POD5 container (a signature,
+ * Arrow IPC files for the Reads, Signal and Run Info tables, a flatbuffer
+ * footer) read in place from a memory map, without the pod5 or Arrow
+ * libraries.  Signal rows are int16 lists or VBZ blobs (zstd frame of a
+ * 1-bit-key stream-VByte block of zig-zag differences, sample count in the
+ * row's `samples` column); a read's signal is its rows in list order.
+ * Arrow body compression: none or ZSTD.  Host-side; a handle is read-only
+ * after open and may be shared by threads. */
+typedef struct dtwax_pod5 dtwax_pod5;
+typedef struct {
+  int64_t n_samples;          /* samples over all the read's rows */
+  int64_t start_sample;       /* `start` column (0 when absent) */
+  int32_t channel;            /* -1 when absent */
+  int32_t read_number;        /* -1 when absent */
+  int32_t n_rows;             /* signal-table rows holding the read */
+  int32_t well;               /* -1 when absent */
+  float calibration_offset;   /* pA = (raw + offset) * scale; NaN when absent */
+  float calibration_scale;
+  char read_id[40];           /* canonical lower-case UUID text, NUL-terminated */
+} dtwax_pod5_read_info;
+/* Map and index a file: both tables' schemas and batches are checked, every
+ * read's row list is checked against the signal table, and a `num_samples`
+ * column must agree with the rows' `samples`.  DTWAX_EIO on any mismatch. */
+int dtwax_pod5_open(const char* path, dtwax_pod5** out);
+int dtwax_pod5_close(dtwax_pod5* f);
+int dtwax_pod5_count(const dtwax_pod5* f, int64_t* n_reads, int64_t* n_signal_rows);
+/* Metadata of reads [first, first + n) in table order. */
+int dtwax_pod5_reads(const dtwax_pod5* f, int64_t first, int64_t n, dtwax_pod5_read_info* out);
+/* Samples [skip, skip + take) of the n reads listed in `reads` (table
+ * indices), decoded on n_threads host threads (0: up to 16).  take >= 0:
+ * read j goes to dst + j*take; take < 0: whole reads from `skip` on, packed
+ * in list order.  n_out[j] = samples written for read j (fewer than take when
+ * the read is shorter).  Rows wholly outside the window are not decoded. */
+int dtwax_pod5_signal(const dtwax_pod5* f, const int64_t* reads, int64_t n, int64_t skip,
+                      int64_t take, int16_t* dst, int64_t dst_cap, int64_t* n_out,
+                      int32_t n_threads);
+
 /* ---- normalisation -----------------------------------------------------
  * Normalise n_reads device-resident reads (float samples; read i starts at
  * d_offsets[i] and has d_lens[i] samples) into d_out (same layout), with
