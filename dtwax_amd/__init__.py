@@ -30,7 +30,7 @@ class DtwaxError(RuntimeError):
 class ScoreOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("segment", C.c_int32), ("ref_chunk", C.c_int32),
                 ("renorm_drift", C.c_float), ("compat_quirks", C.c_int32),
-                ("moves", C.c_int32), ("waves", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("moves", C.c_int32), ("waves", C.c_int32), ("packing", C.c_int32)]
 
 
 RESULT_DTYPE = np.dtype([("score", np.float32, 2), ("argmin", np.int32, 2)])
@@ -74,6 +74,7 @@ def lib():
                                  C.POINTER(i32), C.POINTER(i32)], C.c_int),
             "dtwax_score_plan": ([vp, vp, vp, vp, vp, vp], C.c_int),
             "dtwax_plan_set_timing": ([vp, i32], C.c_int),
+            "dtwax_plan_packing": ([vp, C.POINTER(i32), C.POINTER(i32)], C.c_int),
             "dtwax_plan_kernel_ms": ([vp, C.POINTER(f32)], C.c_int),
             "dtwax_distances": ([vp, vp, vp, vp, i32, C.c_uint32, vp, vp, i32], C.c_int),
             "dtwax_selftest_lanes": ([vp], C.c_int),
@@ -115,7 +116,7 @@ def exported_symbols():
         "dtwax_event_destroy", "dtwax_event_record", "dtwax_event_elapsed_ms",
         "dtwax_reference_upload", "dtwax_squiggle_build", "dtwax_normalize", "dtwax_score",
         "dtwax_plan_create", "dtwax_plan_destroy", "dtwax_plan_info", "dtwax_score_plan",
-        "dtwax_plan_set_timing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
+        "dtwax_plan_set_timing", "dtwax_plan_packing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
         "dtwax_autotune", "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
         "dtwax_ru_release", "dtwax_ru_start", "dtwax_ru_submit", "dtwax_ru_poll", "dtwax_ru_stop",
         "dtwax_vbz_decode", "dtwax_pod5_open", "dtwax_pod5_close", "dtwax_pod5_count",
@@ -176,11 +177,11 @@ def pack_reads(reads):
     return np.ascontiguousarray(flat, np.float32), offs, lens
 
 
-def make_opts(precision=F16, segment=0, ref_chunk=0, renorm_drift=0.0, moves=2, waves=0):
+def make_opts(precision=F16, segment=0, ref_chunk=0, renorm_drift=0.0, moves=2, waves=0, packing=0):
     if isinstance(precision, str):
         precision = PRECISIONS[precision]
     return ScoreOpts(int(precision), int(segment), int(ref_chunk), float(renorm_drift), 0,
-                     int(moves), int(waves))
+                     int(moves), int(waves), int(packing))
 
 
 class Plan:
@@ -199,8 +200,11 @@ class Plan:
         n_items, seg, chunk, grid = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         _check(lib().dtwax_plan_info(self.handle, C.byref(cells), C.byref(comp), C.byref(n_items),
                                      C.byref(seg), C.byref(chunk), C.byref(grid)))
+        packs, packed = C.c_int32(), C.c_int32()
+        _check(lib().dtwax_plan_packing(self.handle, C.byref(packs), C.byref(packed)))
         return dict(cells=cells.value, computed=comp.value, n_items=n_items.value,
-                    segment=seg.value, ref_chunk=chunk.value, grid=grid.value)
+                    segment=seg.value, ref_chunk=chunk.value, grid=grid.value,
+                    packs=packs.value, packed_reads=packed.value)
 
     def run(self, d_queries, d_offsets, d_lens, d_out, stream=None):
         _check(lib().dtwax_score_plan(self.handle, d_queries.ptr, d_offsets.ptr, d_lens.ptr,

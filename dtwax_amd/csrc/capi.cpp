@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <set>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -322,36 +323,143 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
       seg_of[(size_t)r] = cand[best];
     }
   }
+  // Packing: a read's rows as a partial band (its length modulo 64 x SP
+  // rows, padded at the front to a lane) and full bands after it; the partial
+  // bands of several reads share one band on consecutive lanes, so a read
+  // pads fewer than SP rows instead of up to a band.  The shared band costs
+  // ~2% more per cell (lane masks, history flushes).  Automatic: the reads
+  // whose packed cost beats their unpacked one by 1%, kept only if the
+  // packed bands as filled still beat the unpacked cost of those reads.
+  if (o.packing < -1 || o.packing > 1) return fail(DTWAX_EINVAL, "packing must be -1, 0 or 1");
+  const bool pack_form = !m3 && (o.precision == DTWAX_F16 || o.precision == DTWAX_F16_FUSED);
+  if (o.packing == 1 && (!pack_form || (o.segment && o.segment != 16 && o.segment != 32)))
+    return fail(DTWAX_EINVAL, "packing needs DTWAX_F16 or DTWAX_F16_FUSED, two moves and segment 0, 16 or 32");
+  const int SP = (o.segment == 16 || o.segment == 32) ? o.segment : 32;
+  const int64_t BRP = 64 * SP;
+  struct Seg {
+    int32_t read, lanes, nfull, rp;
+  };
+  std::vector<Seg> segs;
+  if (pack_form && (o.packing == 1 || (o.packing == 0 && !o.segment))) {
+    static const int cand[7] = {8, 12, 16, 20, 25, 28, 32};
+    static const double f0[7] = {1.03, 1.011, 1.011, 1.0, 1.0, 1.0, 1.0};
+    for (int32_t r = 0; r < n_reads; r++) {
+      const int64_t q = lens[r];
+      const int32_t nfull = (int32_t)((q - 1) / BRP);
+      const int32_t rp = (int32_t)(q - nfull * BRP);
+      const int32_t lanes = (rp + SP - 1) / SP;
+      if (lanes < 4) continue;  // the history flush period is at least 4 steps
+      if (o.packing == 0) {
+        const int ci = (int)(std::find(cand, cand + 7, seg_of[(size_t)r]) - cand);
+        const int64_t BR = 64 * cand[ci];
+        const int64_t nb = (q + BR - 1) / BR;
+        const double un = (double)(nb * BR) * f0[ci] * (1.0 + 0.015 * (double)(nb - 1));
+        const double pk = ((double)(nfull * BRP) + 1.02 * lanes * SP) * (1.0 + 0.015 * nfull);
+        if (pk >= 0.99 * un) continue;
+      }
+      segs.push_back({r, lanes, nfull, rp});
+    }
+  }
+  // best fit decreasing into 64-lane bands, at most 4 continuing reads each
+  // (each needs a scratch row)
+  struct Bin {
+    int32_t used = 0, nmulti = 0;
+    std::vector<int32_t> segs;
+  };
+  std::vector<Bin> bins;
+  if (!segs.empty()) {
+    std::stable_sort(segs.begin(), segs.end(), [](const Seg& a, const Seg& b) { return a.lanes > b.lanes; });
+    std::set<std::pair<int32_t, int32_t>> open_any, open_final;  // (free lanes, bin)
+    for (int32_t i = 0; i < (int32_t)segs.size(); i++) {
+      const Seg& sg = segs[(size_t)i];
+      const bool multi = sg.nfull > 0;
+      int32_t b = -1;
+      const auto ia = open_any.lower_bound({sg.lanes, -1});
+      const auto jf = multi ? open_final.end() : open_final.lower_bound({sg.lanes, -1});
+      const bool has_a = ia != open_any.end(), has_f = jf != open_final.end();
+      if (has_f && (!has_a || jf->first < ia->first)) {
+        b = jf->second;
+        open_final.erase(jf);
+      } else if (has_a) {
+        b = ia->second;
+        open_any.erase(ia);
+      } else {
+        b = (int32_t)bins.size();
+        bins.emplace_back();
+      }
+      Bin& bn = bins[(size_t)b];
+      bn.used += sg.lanes;
+      bn.nmulti += multi ? 1 : 0;
+      bn.segs.push_back(i);
+      if (bn.used < 64) (bn.nmulti < 4 ? open_any : open_final).insert({64 - bn.used, b});
+    }
+    if (o.packing == 0) {
+      // keep the packing only if the filled bands still gain
+      double packed = 0, unpacked = 0;
+      static const int cand[7] = {8, 12, 16, 20, 25, 28, 32};
+      static const double f0[7] = {1.03, 1.011, 1.011, 1.0, 1.0, 1.0, 1.0};
+      for (const Seg& sg : segs) {
+        const int64_t q = lens[sg.read];
+        const int ci = (int)(std::find(cand, cand + 7, seg_of[(size_t)sg.read]) - cand);
+        const int64_t BR = 64 * cand[ci];
+        const int64_t nb = (q + BR - 1) / BR;
+        unpacked += (double)(nb * BR) * f0[ci] * (1.0 + 0.015 * (double)(nb - 1));
+        packed += (double)(sg.nfull * BRP) * (1.0 + 0.015 * sg.nfull);
+      }
+      packed += 1.02 * (double)BRP * (double)bins.size();
+      if (packed >= 0.99 * unpacked) {
+        segs.clear();
+        bins.clear();
+      }
+    }
+  }
+  std::vector<char> packed_read((size_t)n_reads, 0);
+  for (const Seg& sg : segs) packed_read[(size_t)sg.read] = 1;
+
   std::vector<int> widths;
   for (int32_t r = 0; r < n_reads; r++)
-    if (std::find(widths.begin(), widths.end(), seg_of[(size_t)r]) == widths.end()) widths.push_back(seg_of[(size_t)r]);
+    if (!packed_read[(size_t)r] && std::find(widths.begin(), widths.end(), seg_of[(size_t)r]) == widths.end())
+      widths.push_back(seg_of[(size_t)r]);
   std::sort(widths.rbegin(), widths.rend());
-  if (widths.empty()) widths.push_back(o.segment ? o.segment : 32);
-  p->S = widths.size() == 1 ? widths[0] : 0;
-  p->renorm_k = drift * (float)widths[0];
+  if (widths.empty() && bins.empty()) widths.push_back(o.segment ? o.segment : 32);
+  if (bins.empty()) {
+    p->S = widths.size() == 1 ? widths[0] : 0;
+  } else {
+    p->S = (widths.empty() || (widths.size() == 1 && widths[0] == SP)) ? SP : 0;
+  }
+  p->renorm_k = drift * (float)(widths.empty() ? SP : widths[0]);
 
   bool multiband = false;
+  int32_t pack_rows = 1;  // band scratch rows per workgroup of the packed launch
+  for (const Bin& bn : bins) pack_rows = std::max(pack_rows, bn.nmulti);
   int64_t slots = 1, slots_sum = 0;
-  for (int32_t r = 0; r < n_reads; r++) multiband |= lens[r] > 64 * seg_of[(size_t)r];
+  for (int32_t r = 0; r < n_reads; r++) multiband |= !packed_read[(size_t)r] && lens[r] > 64 * seg_of[(size_t)r];
+  for (const Seg& sg : segs) multiband |= sg.nfull > 0;
   for (int S : widths) {
     const int64_t sl = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, m3)) * c->n_cu;
     slots = std::max(slots, sl);
     slots_sum += sl;
   }
+  const int64_t slots_pack = bins.empty() ? 0 : (int64_t)std::max(1, occupancy_f16_pack(p->precision, SP)) * c->n_cu;
+  if (!bins.empty()) {
+    slots = std::max(slots, slots_pack);
+    slots_sum += slots_pack * pack_rows;
+  }
+  const int64_t n_units = (int64_t)(n_reads - (int32_t)segs.size()) + (int64_t)bins.size();
 
   // chunking: explicit, or whole strands unless the GPU would be underfilled
   // or the band scratch would not fit the budget.
   int64_t chunk = o.ref_chunk ? o.ref_chunk : N;
   if (!o.ref_chunk && !m3) {
-    if ((int64_t)n_reads < slots && n_reads > 0) {
-      int64_t want = (slots + n_reads - 1) / n_reads;
+    if (n_units < slots && n_units > 0) {
+      int64_t want = (slots + n_units - 1) / n_units;
       int64_t ch = (N + want - 1) / want;
       chunk = std::max<int64_t>(ch, std::min<int64_t>(N, 8 * (int64_t)qmax));
     }
     if (multiband) {
       const int64_t budget = scratch_budget();
       const int64_t bytes_per_col = 4 * (p->precision == DTWAX_F32 ? p->nstr : 1);
-      const int64_t max_w = budget / (std::min<int64_t>(slots_sum, (int64_t)n_reads * ((N + chunk - 1) / chunk)) * bytes_per_col);
+      const int64_t max_w = budget / (std::min<int64_t>(slots_sum, n_units * pack_rows * ((N + chunk - 1) / chunk)) * bytes_per_col);
       if (chunk + qmax > max_w) chunk = std::max<int64_t>(4096, max_w - qmax);
     }
   }
@@ -366,7 +474,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     const int64_t BR = 64 * S;
     std::vector<std::pair<int64_t, WorkItem>> items;
     for (int32_t r = 0; r < n_reads; r++) {
-      if (seg_of[(size_t)r] != S) continue;
+      if (seg_of[(size_t)r] != S || packed_read[(size_t)r]) continue;
       const int64_t q = lens[r];
       const int64_t nb = (q + BR - 1) / BR;
       for (int64_t c0 = 0; c0 < N; c0 += chunk) {
@@ -404,6 +512,81 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     for (auto& pr : items) hv.push_back(pr.second);
     p->groups.push_back(g);
   }
+  // packed items: one per band of packed partial bands and chunk
+  std::vector<PackLane> hlanes;
+  std::vector<PackRead> hreads;
+  if (!bins.empty()) {
+    std::vector<std::pair<int64_t, WorkItem>> items;
+    for (int32_t b = 0; b < (int32_t)bins.size(); b++) {
+      const Bin& bn = bins[(size_t)b];
+      const int32_t first_read = (int32_t)hreads.size();
+      int32_t lane = 0, min_lanes = 64, nmulti = 0, nfull_sum = 0;
+      int64_t qb = 1;
+      for (int32_t si : bn.segs) {
+        const Seg& sg = segs[(size_t)si];
+        const int32_t P = sg.lanes * SP - sg.rp;
+        const int32_t endl = lane + sg.lanes - 1;
+        for (int32_t l = 0; l < sg.lanes; l++) {
+          PackLane pl;
+          pl.read = sg.read;
+          pl.r0 = l * SP - P;
+          pl.info = endl | (l == 0 ? PL_START : 0) | (sg.nfull ? PL_HAND : PL_FINAL);
+          pl.row = sg.nfull ? nmulti : -1;
+          hlanes.push_back(pl);
+        }
+        if (sg.nfull) {
+          hreads.push_back({sg.read, sg.nfull, sg.rp, nmulti});
+          nmulti++;
+          nfull_sum += sg.nfull;
+        }
+        lane += sg.lanes;
+        min_lanes = std::min(min_lanes, sg.lanes);
+        qb = std::max<int64_t>(qb, lens[sg.read]);
+      }
+      for (; lane < 64; lane++) hlanes.push_back({-1, -SP, lane, -1});  // idle
+      int32_t F = 64;
+      while (F > min_lanes) F >>= 1;
+      for (int64_t c0 = 0; c0 < N; c0 += chunk) {
+        const int64_t c1 = std::min<int64_t>(N, c0 + chunk);
+        const int64_t x = std::max<int64_t>(0, c0 - (qb - 1));
+        WorkItem it{};
+        it.read = b;
+        it.slot = first_read;
+        it.x = (int32_t)x;
+        it.c0 = (int32_t)c0;
+        it.c1 = (int32_t)c1;
+        it.flags = ITEM_PACK;
+        it.cur = nmulti;
+        it.reserved = F;
+        const int64_t steps = ((c1 - x + 63) + 63) / 64 * 64;
+        items.push_back({(1 + nfull_sum) * steps, it});
+        for (int32_t si : bn.segs) p->cells += (int64_t)lens[segs[(size_t)si].read] * (c1 - c0) * strands_counted;
+        p->computed += (1 + nfull_sum) * BRP * steps * 2;
+        max_w = std::max(max_w, c1 - x);
+      }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<int64_t, WorkItem>& a, const std::pair<int64_t, WorkItem>& b) {
+                       return a.first > b.first;
+                     });
+    LaunchGroup g;
+    g.S = SP;
+    g.pack = true;
+    g.rows = pack_rows;
+    g.item0 = (int32_t)hv.size();
+    g.n_items = (int32_t)items.size();
+    g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(slots_pack, (int64_t)items.size()));
+    if (o.waves > 0) g.grid = std::min(g.grid, o.waves);
+    g.renorm_k = drift * (float)SP;
+    g.scratch_wg0 = scratch_rows;
+    scratch_rows += (int64_t)g.grid * pack_rows;
+    max_grid = std::max(max_grid, g.grid);
+    for (auto& pr : items) hv.push_back(pr.second);
+    // the packed launch goes first: its items are the longest
+    p->groups.insert(p->groups.begin(), g);
+  }
+  p->n_packs = (int32_t)bins.size();
+  p->n_packed_reads = (int32_t)segs.size();
   p->n_items = (int32_t)hv.size();
   if (m3 && multiband) {
     // whole-strand band scratch rows: fewer resident waves if the rows of a
@@ -437,6 +620,14 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   if (hipMalloc(&p->d_keys, std::max<size_t>(1, (size_t)n_reads * 2) * sizeof(unsigned long long)) != hipSuccess)
     return cleanup("key allocation failed");
   if (!hv.empty()) HIP_TRY(hipMemcpy(p->d_items, hv.data(), hv.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  if (!hlanes.empty()) {
+    if (hipMalloc(&p->d_pack_lanes, hlanes.size() * sizeof(PackLane)) != hipSuccess ||
+        hipMalloc(&p->d_pack_reads, std::max<size_t>(1, hreads.size()) * sizeof(PackRead)) != hipSuccess)
+      return cleanup("pack table allocation failed");
+    HIP_TRY(hipMemcpy(p->d_pack_lanes, hlanes.data(), hlanes.size() * sizeof(PackLane), hipMemcpyHostToDevice));
+    if (!hreads.empty())
+      HIP_TRY(hipMemcpy(p->d_pack_reads, hreads.data(), hreads.size() * sizeof(PackRead), hipMemcpyHostToDevice));
+  }
   if (p->groups.size() > 1) {
     if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess)
       return cleanup("event creation failed");
@@ -464,7 +655,8 @@ void dtwax_plan_destroy(dtwax_plan* p) {
   if (!p) return;
   (void)hipSetDevice(p->ctx->device);
   (void)hipDeviceSynchronize();
-  for (void* q : {(void*)p->d_items, (void*)p->d_counter, (void*)p->d_keys, p->d_scratch})
+  for (void* q : {(void*)p->d_items, (void*)p->d_counter, (void*)p->d_keys, p->d_scratch, (void*)p->d_pack_lanes,
+                  (void*)p->d_pack_reads})
     if (q) (void)hipFree(q);
   for (auto& g : p->groups) {
     if (g.stream) (void)hipStreamDestroy(g.stream);
@@ -496,6 +688,13 @@ int dtwax_plan_set_timing(dtwax_plan* p, int32_t enable) {
     HIP_TRY(hipEventCreate(&p->ev1));
   }
   p->timing = enable != 0;
+  return DTWAX_OK;
+}
+
+int dtwax_plan_packing(const dtwax_plan* p, int32_t* n_packs, int32_t* n_packed_reads) {
+  if (!p) return fail(DTWAX_EINVAL, "NULL plan");
+  if (n_packs) *n_packs = p->n_packs;
+  if (n_packed_reads) *n_packed_reads = p->n_packed_reads;
   return DTWAX_OK;
 }
 
@@ -554,7 +753,11 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
       k.counter = L.counter + g;
       k.renorm_k = G.renorm_k;
       k.scratch = L.scratch ? (void*)((float*)L.scratch + G.scratch_wg0 * L.scratch_stride * epc) : nullptr;
-      HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, std::max(1, std::min(G.grid, G.n_items)), L.m3, gs));
+      k.scratch_rows = G.rows;
+      k.pack_lanes = L.pack_lanes;
+      k.pack_reads = L.pack_reads;
+      HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, std::max(1, std::min(G.grid, G.n_items)), L.m3, gs,
+                          G.pack));
     }
     if (gs != st) {
       HIP_TRY(hipEventRecord(G.done, gs));
@@ -589,6 +792,8 @@ static int run_plan(dtwax_plan* p, const float* d_q, const int64_t* d_off, const
   L.renorm_k = p->renorm_k;
   L.groups = p->groups.data();
   L.n_groups = (int)p->groups.size();
+  L.pack_lanes = p->d_pack_lanes;
+  L.pack_reads = p->d_pack_reads;
   L.fork = p->fork;
   L.m3 = p->m3;
   L.out = reinterpret_cast<DeviceResult*>(d_out);

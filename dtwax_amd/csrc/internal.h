@@ -27,7 +27,9 @@ struct LaunchGroup {
   int32_t n_items = 0;
   int grid = 1;
   float renorm_k = 0.f;
-  int64_t scratch_wg0 = 0;  // first band-scratch row (workgroup slot) of this group
+  int64_t scratch_wg0 = 0;  // first band-scratch row of this group
+  bool pack = false;        // packed items (sdtw_f16_pack_kernel)
+  int32_t rows = 1;         // band scratch rows per workgroup
   hipStream_t stream = nullptr;  // groups after the first run on their own stream
   hipEvent_t done = nullptr;     // recorded on that stream after the launch
 };
@@ -57,6 +59,8 @@ struct ScoreLaunch {
   bool m3 = false;  // three moves
   DeviceResult* out = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const PackLane* pack_lanes = nullptr;  // packed groups' lane and read tables
+  const PackRead* pack_reads = nullptr;
   const LaunchGroup* groups = nullptr;  // counter[g] for group g
   int n_groups = 0;
   hipEvent_t fork = nullptr;  // recorded on the caller's stream before the groups start
@@ -101,6 +105,9 @@ struct dtwax_plan {
   int64_t cells = 0, computed = 0;
   float renorm_k = 0.f;
   dtwax::WorkItem* d_items = nullptr;
+  dtwax::PackLane* d_pack_lanes = nullptr;
+  dtwax::PackRead* d_pack_reads = nullptr;
+  int32_t n_packs = 0, n_packed_reads = 0;
   int32_t* d_counter = nullptr;
   unsigned long long* d_keys = nullptr;
   void* d_scratch = nullptr;

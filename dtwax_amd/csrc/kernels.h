@@ -34,6 +34,33 @@ struct WorkItem {
 // store the new last row into the slot's other row.
 constexpr int32_t ITEM_RESUME = 1;
 constexpr int32_t ITEM_STORE = 2;
+// A packed item: `read` is a pack, whose shared band holds the first
+// (partial) band of several reads on consecutive lanes; `slot` is its first
+// PackRead, `cur` the number of PackReads (reads with full bands after the
+// shared band), `reserved` the shared band's history flush period in steps
+// (a power of two, at most the shortest segment's lane count).
+constexpr int32_t ITEM_PACK = 4;
+
+// One lane of a pack's shared band.  A read's rows [0, rp) fill lanes
+// [start, end] with the rows padded at the front to a lane boundary
+// (virtual rows, zero cost).
+struct PackLane {
+  int32_t read;  // -1: idle lane
+  int32_t r0;    // read row of the lane's first row (< 0: virtual rows first)
+  int32_t info;  // segment end lane (bits 0-7) | PL_* flags
+  int32_t row;   // band scratch row of the read when PL_HAND, else -1
+};
+constexpr int32_t PL_START = 256;  // first lane of a read: top input is the free start
+constexpr int32_t PL_FINAL = 512;  // the read ends in this band: its min is scored here
+constexpr int32_t PL_HAND = 1024;  // the read continues: its last row goes to `row`
+
+// A packed read with full bands after the shared band: rows rp + b * 64S ...
+struct PackRead {
+  int32_t read;
+  int32_t nfull;  // full bands after the shared band
+  int32_t rp;     // its rows in the shared band
+  int32_t row;    // band scratch row (within the wave's rows)
+};
 
 struct DeviceResult {
   float score[2];
@@ -56,11 +83,15 @@ struct KParams {
                              // starts (2*s + k) * row_stride columns further
   int64_t row_stride;        // columns per state row (includes the pads)
   float renorm_k;            // F16_RENORM per-lane drift constant (S rows' worth)
+  const PackLane* pack_lanes;  // 64 per pack (packed launches)
+  const PackRead* pack_reads;
+  int32_t scratch_rows;        // band scratch rows per workgroup (packed launches; else 1)
 };
 
 // m3: three moves (the left move D[r][c-1] too; DTWAX_F16 and DTWAX_F32 only)
+// pack: the packed kernel (DTWAX_F16 / DTWAX_F16_FUSED, S = 16 or 32, two moves)
 hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool m3,
-                       hipStream_t st);
+                       hipStream_t st, bool pack = false);
 // zero n_counters work counters (n_counters <= 256) and set n_keys keys to ~0
 hipError_t launch_reset(int32_t* counters, int32_t n_counters, unsigned long long* keys, int64_t n_keys,
                         hipStream_t st);
@@ -68,6 +99,7 @@ hipError_t launch_finalize(const unsigned long long* keys, int32_t n_reads, int3
                            DeviceResult* out, hipStream_t st);
 hipError_t launch_selftest_lanes(int32_t* d_out, hipStream_t st);
 int occupancy_f16(int precision, int S, bool m3);
+int occupancy_f16_pack(int precision, int S);
 int occupancy_f32(int S, int nstr, bool m3);
 
 // normalisation (normalize_kernels.hip)

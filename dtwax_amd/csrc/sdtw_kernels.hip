@@ -486,6 +486,112 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
   }
 }
 
+// The shared band of a pack: f16_band2 with every read's lanes independent.
+// A read's first lane takes its top inputs from ring_t at its own columns
+// (0 from the item's first column on, +inf before it: the free start that
+// lane 0 of an unpacked band has), and its last lane inserts its last row
+// into the read's history chain (lane 63 alone does so in an unpacked band).
+// The chains are flushed every F steps, F at most the shortest read's lane
+// count, so no value leaves its read's lanes unseen: PL_FINAL lanes fold the
+// values into best/pos, PL_HAND lanes write them to the read's scratch row
+// `hrow` for its full bands.  Lane L of a read ending at lane E holds, after
+// step t, the value E made at step t - (E - L): column x + 2t - 4E + 2L (A).
+template <int S, int MODE>
+__device__ __forceinline__ void f16_band2_pack(const F16Band<S, MODE>& B, const SweepIO& io, h2* ring_s,
+                                               h2* ring_t, int lane, int32_t info, h2* hrow, int32_t F,
+                                               h2& best, int32_t& posf, int32_t& posr) {
+  const h2 INF = hsplat((half_t)__builtin_inff());
+  const h2 ZERO = hsplat((half_t)0.0f);
+  const int32_t x = io.x, c0 = io.c0, c1 = io.c1;
+  const h2* __restrict__ ref = io.ref;
+  const bool start = (info & PL_START) != 0;
+  const bool end = (info & 255) == lane;
+  const bool fin = (info & PL_FINAL) != 0;
+  const bool hand = (info & PL_HAND) != 0;
+  const int32_t cst = x + 2 * lane - 4 * (info & 255);
+  auto put = [&](h2* ring, int slot, h2 v) {
+    ring[slot] = v;
+    if (slot < MIRROR2) ring[RING2 + slot] = v;
+  };
+  put(ring_s, lane, ZERO);
+  put(ring_s, 64 + lane, ZERO);
+  put(ring_t, lane, INF);
+  put(ring_t, 64 + lane, INF);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int o = 64 * k + lane;
+    if (o < 130) {
+      put(ring_s, o + 128, ref[x + o]);
+      put(ring_t, o + 128, ZERO);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+
+  h2 Bc[S];
+#pragma unroll
+  for (int i = 0; i < S; i++) Bc[i] = INF;
+  h2 tinA = INF, tinB = INF, diagA = INF, histA = INF, histB = INF;
+  if (lane == 0) {
+    tinA = ring_t[128];
+    tinB = ring_t[129];
+  }
+  const int base0 = (128 - 2 * lane) & (RING2 - 1);
+  h2 sA = ring_s[base0], sB = ring_s[base0 + 1];
+
+  auto flush = [&](int32_t t) {
+    const int32_t colA = cst + 2 * t;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int32_t col = colA + h;
+      const h2 v = h ? histB : histA;
+      if (fin && col >= c0 && col < c1) {
+        if (v.x < best.x) { best.x = v.x; posf = col; }
+        if (v.y < best.y) { best.y = v.y; posr = col; }
+      }
+      if (hand && col >= x && col < c1) hrow[col] = v;
+    }
+  };
+
+  const int32_t nsteps = (((c1 - x + 127) >> 1) + 63) & ~63;
+  for (int32_t T = 0; T < nsteps; T += 64) {
+    h2 pf_s[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) pf_s[h] = ref[x + 2 * T + 130 + 64 * h + lane];
+    const int base = (2 * T + 128 - 2 * lane) & (RING2 - 1);
+#pragma unroll 1
+    for (int g = 0; g < 64; g += 4) {
+      const h2* rs = ring_s + base + 2 * g;
+      const h2* rt = ring_t + base + 2 * g;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const h2 nA = rs[2 * u + 2], nB = rs[2 * u + 3];
+        const h2 tfA = rt[2 * u + 2], tfB = rt[2 * u + 3];
+        h2 lastA;
+        f16_cells2<S, MODE>(B, sA, sB, tinA, tinB, diagA, Bc, lastA);
+        diagA = tinB;
+        const h2 shA = dpp_shr1(tfA, lastA), shB = dpp_shr1(tfB, Bc[S - 1]);
+        tinA = start ? tfA : shA;
+        tinB = start ? tfB : shB;
+        const h2 hA = dpp_shl1(lastA, histA), hB = dpp_shl1(Bc[S - 1], histB);
+        histA = end ? lastA : hA;
+        histB = end ? Bc[S - 1] : hB;
+        sA = nA;
+        sB = nB;
+      }
+      if (((g + 4) & (F - 1)) == 0) flush(T + g + 3);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int slot = (2 * T + 258 + 64 * h + lane) & (RING2 - 1);
+      put(ring_s, slot, pf_s[h]);
+      put(ring_t, slot, ZERO);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 __device__ __forceinline__ half_t clamp_half(float v) { return (half_t)v; }
 
 template <int S, int MODE, bool M3>
@@ -595,6 +701,99 @@ __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
     if (idx >= p.n_items) break;
     WorkItem it = p.items[idx];
     f16_item<S, MODE, M3>(p, it, lane, ring_s, ring_t, scratch);
+  }
+}
+
+// A pack: the shared band (the first, partial band of each of its reads),
+// the scores of the reads that end there, then each continuing read's full
+// bands in order, from its scratch row, as an unpacked item's bands.
+template <int S, int MODE>
+__device__ void f16_pack_item(const KParams& p, const WorkItem& it, int lane, h2* ring_s, h2* ring_t,
+                              h2* scratch_w) {
+  constexpr int BR = 64 * S;
+  const h2 INF = hsplat((half_t)__builtin_inff());
+  const PackLane pl = p.pack_lanes[64 * (int64_t)it.read + lane];
+  SweepIO io;
+  io.ref = (const h2*)p.ref;
+  io.trow = nullptr;
+  io.orow = nullptr;
+  io.x = it.x;
+  io.c0 = it.c0;
+  io.c1 = it.c1;
+  io.final_band = true;
+  io.top_diag = false;
+  h2 best = INF;
+  int32_t posf = -1, posr = -1;
+  {
+    F16Band<S, MODE> B;
+    const float* __restrict__ qp = p.queries + (pl.read >= 0 ? p.q_off[pl.read] : 0);
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+      const int32_t r = pl.r0 + i;
+      if (pl.read >= 0 && r >= 0) {
+        half_t q = (half_t)__builtin_nontemporal_load(&qp[r]);
+        B.Qn[i] = hsplat(-q);
+        B.A[i] = hsplat((half_t)1.0f);
+      } else {
+        B.Qn[i] = hsplat((half_t)0.0f);
+        B.A[i] = hsplat((half_t)0.0f);
+      }
+    }
+    h2* hrow = (pl.info & PL_HAND) ? scratch_w + (int64_t)pl.row * p.scratch_stride - it.x : nullptr;
+    f16_band2_pack<S, MODE>(B, io, ring_s, ring_t, lane, pl.info, hrow, it.reserved, best, posf, posr);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __threadfence_block();
+  {
+    // every lane issues the atomics; lanes without a score pass the identity
+    const bool fin = (pl.info & PL_FINAL) != 0;
+    const uint64_t kf = (fin && posf >= 0) ? make_key((float)best.x, posf) : ~0ull;
+    const uint64_t kr = (fin && posr >= 0) ? make_key((float)best.y, posr) : ~0ull;
+    unsigned long long* kp = &p.keys[2 * (int64_t)(pl.read >= 0 ? pl.read : 0)];
+    atomicMin(kp, (unsigned long long)kf);
+    atomicMin(kp + 1, (unsigned long long)kr);
+  }
+  for (int32_t j = 0; j < it.cur; j++) {
+    const PackRead pr = p.pack_reads[it.slot + j];
+    const float* __restrict__ qp = p.queries + p.q_off[pr.read];
+    h2* scr = scratch_w + (int64_t)pr.row * p.scratch_stride - it.x;
+    best = INF;
+    posf = posr = -1;
+    for (int32_t b = 0; b < pr.nfull; b++) {
+      F16Band<S, MODE> B;
+      const int32_t r0 = pr.rp + b * BR + lane * S;
+#pragma unroll
+      for (int i = 0; i < S; i++) {
+        half_t q = (half_t)__builtin_nontemporal_load(&qp[r0 + i]);
+        B.Qn[i] = hsplat(-q);
+        B.A[i] = hsplat((half_t)1.0f);
+      }
+      io.trow = scr;
+      io.final_band = (b == pr.nfull - 1);
+      io.orow = io.final_band ? nullptr : scr;
+      f16_band2<S, MODE>(B, io, ring_s, ring_t, lane, best, posf, posr);
+      __builtin_amdgcn_s_waitcnt(0);
+      __threadfence_block();
+    }
+    const uint64_t kf = (posf >= 0) ? make_key((float)best.x, posf) : ~0ull;
+    const uint64_t kr = (posr >= 0) ? make_key((float)best.y, posr) : ~0ull;
+    publish_min(&p.keys[2 * (int64_t)pr.read], wave_min_u64(kf), lane);
+    publish_min(&p.keys[2 * (int64_t)pr.read + 1], wave_min_u64(kr), lane);
+  }
+}
+
+template <int S, int MODE>
+__global__ __launch_bounds__(64) void sdtw_f16_pack_kernel(KParams p) {
+  __shared__ h2 ring_s[RING2_ALLOC];
+  __shared__ h2 ring_t[RING2_ALLOC];
+  const int lane = threadIdx.x;
+  h2* scratch_w = p.scratch ? (h2*)p.scratch + (int64_t)blockIdx.x * p.scratch_rows * p.scratch_stride + SCRATCH_PADL
+                            : nullptr;
+  for (;;) {
+    const int idx = fetch_item(p.counter, lane);
+    if (idx >= p.n_items) break;
+    WorkItem it = p.items[idx];
+    f16_pack_item<S, MODE>(p, it, lane, ring_s, ring_t, scratch_w);
   }
 }
 
@@ -907,8 +1106,22 @@ static hipError_t launch_f16_s(const KParams& p, int S, int grid, bool m3, hipSt
   return hipErrorInvalidValue;
 }
 
+template <int MODE>
+static hipError_t launch_f16_pack(const KParams& p, int S, int grid, hipStream_t st) {
+  if (S == 32) hipLaunchKernelGGL((sdtw_f16_pack_kernel<32, MODE>), dim3(grid), dim3(64), 0, st, p);
+  else if (S == 16) hipLaunchKernelGGL((sdtw_f16_pack_kernel<16, MODE>), dim3(grid), dim3(64), 0, st, p);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool m3,
-                       hipStream_t st) {
+                       hipStream_t st, bool pack) {
+  if (pack) {
+    if (m3) return hipErrorInvalidValue;
+    if (precision == DTWAX_F16) return launch_f16_pack<DTWAX_F16>(p, S, grid, st);
+    if (precision == DTWAX_F16_FUSED) return launch_f16_pack<DTWAX_F16_FUSED>(p, S, grid, st);
+    return hipErrorInvalidValue;
+  }
   switch (precision) {
     case DTWAX_F16: return launch_f16_s<DTWAX_F16>(p, S, grid, m3, st);
     case DTWAX_F16_FUSED: return launch_f16_s<DTWAX_F16_FUSED>(p, S, grid, m3, st);
@@ -980,6 +1193,20 @@ int occupancy_f16(int precision, int S, bool m3) {
   }
 #undef OCC_S
 #undef OCC
+  return n;
+}
+
+int occupancy_f16_pack(int precision, int S) {
+  int n = 0;
+  if (precision == DTWAX_F16) {
+    if (S == 32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<32, DTWAX_F16>, 64, 0);
+    if (S == 16) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<16, DTWAX_F16>, 64, 0);
+  } else if (precision == DTWAX_F16_FUSED) {
+    if (S == 32)
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<32, DTWAX_F16_FUSED>, 64, 0);
+    if (S == 16)
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<16, DTWAX_F16_FUSED>, 64, 0);
+  }
   return n;
 }
 

@@ -98,7 +98,13 @@ typedef struct {
                             4096-read batch at 2 waves per SIMD ran 1.3% faster alone
                             and 26% slower right after another kernel), so automatic
                             is the safe choice */
-  int32_t reserved[1];
+  int32_t packing;       /* DTWAX_F16 / DTWAX_F16_FUSED, two moves: 1 = pack every read whose
+                            first band spans at least 4 lanes, 0 = automatic (with segment 0:
+                            the reads that gain), -1 = never.  A packed read's rows start
+                            with a partial band (its length modulo 64 x S rows, padded to a
+                            lane) that shares one band with other reads' partial bands,
+                            followed by its full bands; S = 32, or the segment when 16 or
+                            32.  Scores are identical to the unpacked layout's */
 } dtwax_score_opts;
 
 /* ---- context ----------------------------------------------------------- */
@@ -236,6 +242,9 @@ int dtwax_score_plan(dtwax_plan* plan, const float* d_queries, const int64_t* d_
 /* Enable HIP-event timing of the sDTW kernel launches inside
  * dtwax_score_plan; dtwax_plan_kernel_ms then waits for and returns the last
  * call's sDTW time (all of its launches). */
+/* Packed layout of a plan: bands shared by several reads' partial bands,
+ * and the reads placed in them (0, 0 when the plan packs nothing). */
+int dtwax_plan_packing(const dtwax_plan* plan, int32_t* n_packs, int32_t* n_packed_reads);
 int dtwax_plan_set_timing(dtwax_plan* plan, int32_t enable);
 int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
 
