@@ -228,7 +228,8 @@ def main_c4(a):
                                "10% target, median/MAD, autotuned",
                    "reads": len(lens), "mean_query_len": round(float(lens.mean()), 1),
                    "ref_cols_per_strand": int(n_cols), "segment": pinfo["segment"],
-                   "ref_chunk": pinfo["ref_chunk"], "work_items": pinfo["n_items"], "grid": pinfo["grid"]},
+                   "ref_chunk": pinfo["ref_chunk"], "work_items": pinfo["n_items"], "grid": pinfo["grid"],
+                   "packed_bands": pinfo["packs"], "packed_reads": pinfo["packed_reads"]},
         "reads_classified_per_s": round(len(lens) / (step_ms * 1e-3), 1),
         "kernel_ms": round(km, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(peak_ops / 1e12, 2),

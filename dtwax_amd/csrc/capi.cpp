@@ -360,8 +360,9 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
       segs.push_back({r, lanes, nfull, rp});
     }
   }
-  // best fit decreasing into 64-lane bands, at most 4 continuing reads each
-  // (each needs a scratch row)
+  // best fit decreasing into 64-lane bands, at most 2 continuing reads each
+  // (each needs a band scratch row per wave; a third one fills the bands no
+  // better on uneven batches and would halve the chunk the budget allows)
   struct Bin {
     int32_t used = 0, nmulti = 0;
     std::vector<int32_t> segs;
@@ -391,7 +392,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
       bn.used += sg.lanes;
       bn.nmulti += multi ? 1 : 0;
       bn.segs.push_back(i);
-      if (bn.used < 64) (bn.nmulti < 4 ? open_any : open_final).insert({64 - bn.used, b});
+      if (bn.used < 64) (bn.nmulti < 2 ? open_any : open_final).insert({64 - bn.used, b});
     }
     if (o.packing == 0) {
       // keep the packing only if the filled bands still gain
@@ -520,7 +521,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     for (int32_t b = 0; b < (int32_t)bins.size(); b++) {
       const Bin& bn = bins[(size_t)b];
       const int32_t first_read = (int32_t)hreads.size();
-      int32_t lane = 0, min_lanes = 64, nmulti = 0, nfull_sum = 0;
+      int32_t lane = 0, min_lanes = 64, nmulti = 0;
       int64_t qb = 1;
       for (int32_t si : bn.segs) {
         const Seg& sg = segs[(size_t)si];
@@ -537,13 +538,15 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
         if (sg.nfull) {
           hreads.push_back({sg.read, sg.nfull, sg.rp, nmulti});
           nmulti++;
-          nfull_sum += sg.nfull;
         }
         lane += sg.lanes;
         min_lanes = std::min(min_lanes, sg.lanes);
         qb = std::max<int64_t>(qb, lens[sg.read]);
       }
       for (; lane < 64; lane++) hlanes.push_back({-1, -SP, lane, -1});  // idle
+      std::vector<std::pair<int64_t, int32_t>> multi;  // (length, full bands) of continuing reads
+      for (int32_t si : bn.segs)
+        if (segs[(size_t)si].nfull) multi.push_back({lens[segs[(size_t)si].read], segs[(size_t)si].nfull});
       int32_t F = 64;
       while (F > min_lanes) F >>= 1;
       for (int64_t c0 = 0; c0 < N; c0 += chunk) {
@@ -559,9 +562,15 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
         it.cur = nmulti;
         it.reserved = F;
         const int64_t steps = ((c1 - x + 63) + 63) / 64 * 64;
-        items.push_back({(1 + nfull_sum) * steps, it});
+        // the shared band from x; each read's full bands from its own overlap
+        int64_t band_steps = steps;
+        for (const auto& m : multi) {
+          const int64_t xr = std::max<int64_t>(x, c0 - (m.first - 1));
+          band_steps += (int64_t)m.second * (((c1 - xr + 63) + 63) / 64 * 64);
+        }
+        items.push_back({band_steps, it});
         for (int32_t si : bn.segs) p->cells += (int64_t)lens[segs[(size_t)si].read] * (c1 - c0) * strands_counted;
-        p->computed += (1 + nfull_sum) * BRP * steps * 2;
+        p->computed += BRP * band_steps * 2;
         max_w = std::max(max_w, c1 - x);
       }
     }

@@ -757,6 +757,9 @@ __device__ void f16_pack_item(const KParams& p, const WorkItem& it, int lane, h2
     const PackRead pr = p.pack_reads[it.slot + j];
     const float* __restrict__ qp = p.queries + p.q_off[pr.read];
     h2* scr = scratch_w + (int64_t)pr.row * p.scratch_stride - it.x;
+    // the read's own overlap: a cell that can reach a counted column
+    // (r, c >= c0 - (Q - 1) + r) depends only on columns from c0 - (Q - 1) on
+    io.x = max(it.x, it.c0 - (p.q_len[pr.read] - 1));
     best = INF;
     posf = posr = -1;
     for (int32_t b = 0; b < pr.nfull; b++) {

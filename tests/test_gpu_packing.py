@@ -62,8 +62,8 @@ def test_packed_bit_exact(ctx, precision, chunk):
 
 
 def test_packed_width16_and_scratch_rows(ctx):
-    """S = 16 packs; partial bands of many continuing reads (up to four
-    scratch rows per wave) and flush periods 4, 8, 16, 32 and 64."""
+    """S = 16 packs; partial bands of many continuing reads (two scratch
+    rows per wave) and flush periods 4, 8, 16 and 64."""
     rng = np.random.default_rng(17)
     ref = rng.standard_normal((2, 1800)).astype(np.float32)
     ctx.upload_reference(ref)
