@@ -253,6 +253,104 @@ int dtwax_plan_create(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
 
 }  // extern "C"
 
+namespace {
+// Binary16 segment widths and each one's time per computed cell relative to
+// S = 32, measured on 4096 x 2000-sample batches with the band factor
+// taken out.
+constexpr int kWidths[7] = {8, 12, 16, 20, 25, 28, 32};
+constexpr double kWidthCost[7] = {1.03, 1.011, 1.011, 1.0, 1.0, 1.0, 1.0};
+// The packed layout's shared band: 1.6% more packed ops per cell (lane
+// masks beside the DPP moves) and the history flushes.
+constexpr double kPackCost = 1.02;
+
+// Padded work of a q-row read in bands of 64 x S rows (S one of kWidths):
+// rows x the width's cell time x 1.5% per extra band (hand-over through
+// scratch).
+double band_cost(int64_t q, int S) {
+  const int ci = (int)(std::find(kWidths, kWidths + 7, S) - kWidths);
+  const int64_t BR = 64 * (int64_t)S;
+  const int64_t nb = (q + BR - 1) / BR;
+  return (double)(nb * BR) * kWidthCost[ci] * (1.0 + 0.015 * (double)(nb - 1));
+}
+
+// The width with the least padded work (the narrower one on ties).
+int best_width(int64_t q) {
+  int best = kWidths[0];
+  for (int ci = 1; ci < 7; ci++)
+    if (band_cost(q, kWidths[ci]) < band_cost(q, best) - 1e-9) best = kWidths[ci];
+  return best;
+}
+
+// A packed read: `rp` rows (its length modulo 64 x SP, or a whole band) in
+// `lanes` lanes of a shared band, then `nfull` full bands.
+struct PackSeg {
+  int32_t read, lanes, nfull, rp;
+};
+struct PackBin {
+  int32_t used = 0, nmulti = 0;  // lanes taken; reads continuing past the band
+  std::vector<int32_t> segs;     // indices into the segment list, lane order
+};
+
+// Picks the reads to pack and places their partial bands in 64-lane bands.
+// A read needs at least 4 lanes (the history flush period is at least 4
+// steps).  Automatic: the reads whose packed cost beats their unpacked width
+// by 1%, kept only if the bands as filled still beat those reads' unpacked
+// cost.  Placement: best fit decreasing, at most 2 continuing reads per band
+// (each needs a band scratch row per wave; a third fills the bands no better
+// on uneven batches and would halve the chunk the scratch budget allows).
+void plan_packing(const std::vector<int32_t>& lens, const std::vector<int>& seg_of, int SP, bool automatic,
+                  std::vector<PackSeg>& segs, std::vector<PackBin>& bins) {
+  const int64_t BRP = 64 * (int64_t)SP;
+  for (int32_t r = 0; r < (int32_t)lens.size(); r++) {
+    const int64_t q = lens[(size_t)r];
+    const int32_t nfull = (int32_t)((q - 1) / BRP);
+    const int32_t rp = (int32_t)(q - nfull * BRP);
+    const int32_t lanes = (rp + SP - 1) / SP;
+    if (lanes < 4) continue;
+    const double pk = ((double)(nfull * BRP) + kPackCost * lanes * SP) * (1.0 + 0.015 * nfull);
+    if (automatic && pk >= 0.99 * band_cost(q, seg_of[(size_t)r])) continue;
+    segs.push_back({r, lanes, nfull, rp});
+  }
+  if (segs.empty()) return;
+  std::stable_sort(segs.begin(), segs.end(), [](const PackSeg& a, const PackSeg& b) { return a.lanes > b.lanes; });
+  std::set<std::pair<int32_t, int32_t>> open_any, open_final;  // (free lanes, bin)
+  for (int32_t i = 0; i < (int32_t)segs.size(); i++) {
+    const PackSeg& sg = segs[(size_t)i];
+    const bool multi = sg.nfull > 0;
+    int32_t b = -1;
+    const auto ia = open_any.lower_bound({sg.lanes, -1});
+    const auto jf = multi ? open_final.end() : open_final.lower_bound({sg.lanes, -1});
+    const bool has_a = ia != open_any.end(), has_f = jf != open_final.end();
+    if (has_f && (!has_a || jf->first < ia->first)) {
+      b = jf->second;
+      open_final.erase(jf);
+    } else if (has_a) {
+      b = ia->second;
+      open_any.erase(ia);
+    } else {
+      b = (int32_t)bins.size();
+      bins.emplace_back();
+    }
+    PackBin& bn = bins[(size_t)b];
+    bn.used += sg.lanes;
+    bn.nmulti += multi ? 1 : 0;
+    bn.segs.push_back(i);
+    if (bn.used < 64) (bn.nmulti < 2 ? open_any : open_final).insert({64 - bn.used, b});
+  }
+  if (automatic) {
+    double packed = kPackCost * (double)BRP * (double)bins.size(), unpacked = 0;
+    for (const PackSeg& sg : segs) {
+      unpacked += band_cost(lens[(size_t)sg.read], seg_of[(size_t)sg.read]);
+      packed += (double)(sg.nfull * BRP) * (1.0 + 0.015 * sg.nfull);
+    }
+    if (packed >= 0.99 * unpacked) {
+      segs.clear();
+      bins.clear();
+    }
+  }
+}
+}  // namespace
+
 int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, const dtwax_score_opts* opts,
                       int64_t col_limit, dtwax_plan** out, bool dry) {
   if (!c || !out || (!h_lens && n_reads)) return fail(DTWAX_EINVAL, "NULL argument");
@@ -296,11 +394,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   // Segment width per read.  Fixed when given; the renormalised form
   // subtracts each band's exact row minimum, so it defaults to the shortest
   // bands (64 x 8 rows); binary32 takes one width for the batch; the
-  // binary16 forms pick, per read, the width with the least padded work:
-  // padded rows x the time per computed cell of that width relative to
-  // S = 32 x 1.5% per extra band (band hand-over through scratch).  The
-  // width factors are measured on 4096 x 2000-sample batches with the band
-  // factor taken out: 1.03 at S = 8, 1.011 at 12 and 16, 1.0 from 20 up.
+  // binary16 forms pick, per read, the width with the least padded work.
   std::vector<int> seg_of((size_t)n_reads, 0);
   if (o.segment) {
     std::fill(seg_of.begin(), seg_of.end(), o.segment);
@@ -309,113 +403,20 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   } else if (o.precision == DTWAX_F32) {
     std::fill(seg_of.begin(), seg_of.end(), auto_segment(lens));
   } else {
-    static const int cand[7] = {8, 12, 16, 20, 25, 28, 32};
-    static const double f0[7] = {1.03, 1.011, 1.011, 1.0, 1.0, 1.0, 1.0};
-    auto cost = [&](int64_t q, int ci) {
-      const int64_t BR = 64 * cand[ci];
-      const int64_t nb = (q + BR - 1) / BR;
-      return (double)(nb * BR) * f0[ci] * (1.0 + 0.015 * (double)(nb - 1));
-    };
-    for (int32_t r = 0; r < n_reads; r++) {
-      int best = 0;
-      for (int ci = 1; ci < 7; ci++)
-        if (cost(lens[r], ci) < cost(lens[r], best) - 1e-9) best = ci;
-      seg_of[(size_t)r] = cand[best];
-    }
+    for (int32_t r = 0; r < n_reads; r++) seg_of[(size_t)r] = best_width(lens[r]);
   }
-  // Packing: a read's rows as a partial band (its length modulo 64 x SP
-  // rows, padded at the front to a lane) and full bands after it; the partial
-  // bands of several reads share one band on consecutive lanes, so a read
-  // pads fewer than SP rows instead of up to a band.  The shared band costs
-  // ~2% more per cell (lane masks, history flushes).  Automatic: the reads
-  // whose packed cost beats their unpacked one by 1%, kept only if the
-  // packed bands as filled still beat the unpacked cost of those reads.
   if (o.packing < -1 || o.packing > 1) return fail(DTWAX_EINVAL, "packing must be -1, 0 or 1");
   const bool pack_form = !m3 && (o.precision == DTWAX_F16 || o.precision == DTWAX_F16_FUSED);
   if (o.packing == 1 && (!pack_form || (o.segment && o.segment != 16 && o.segment != 32)))
     return fail(DTWAX_EINVAL, "packing needs DTWAX_F16 or DTWAX_F16_FUSED, two moves and segment 0, 16 or 32");
   const int SP = (o.segment == 16 || o.segment == 32) ? o.segment : 32;
   const int64_t BRP = 64 * SP;
-  struct Seg {
-    int32_t read, lanes, nfull, rp;
-  };
-  std::vector<Seg> segs;
-  if (pack_form && (o.packing == 1 || (o.packing == 0 && !o.segment))) {
-    static const int cand[7] = {8, 12, 16, 20, 25, 28, 32};
-    static const double f0[7] = {1.03, 1.011, 1.011, 1.0, 1.0, 1.0, 1.0};
-    for (int32_t r = 0; r < n_reads; r++) {
-      const int64_t q = lens[r];
-      const int32_t nfull = (int32_t)((q - 1) / BRP);
-      const int32_t rp = (int32_t)(q - nfull * BRP);
-      const int32_t lanes = (rp + SP - 1) / SP;
-      if (lanes < 4) continue;  // the history flush period is at least 4 steps
-      if (o.packing == 0) {
-        const int ci = (int)(std::find(cand, cand + 7, seg_of[(size_t)r]) - cand);
-        const int64_t BR = 64 * cand[ci];
-        const int64_t nb = (q + BR - 1) / BR;
-        const double un = (double)(nb * BR) * f0[ci] * (1.0 + 0.015 * (double)(nb - 1));
-        const double pk = ((double)(nfull * BRP) + 1.02 * lanes * SP) * (1.0 + 0.015 * nfull);
-        if (pk >= 0.99 * un) continue;
-      }
-      segs.push_back({r, lanes, nfull, rp});
-    }
-  }
-  // best fit decreasing into 64-lane bands, at most 2 continuing reads each
-  // (each needs a band scratch row per wave; a third one fills the bands no
-  // better on uneven batches and would halve the chunk the budget allows)
-  struct Bin {
-    int32_t used = 0, nmulti = 0;
-    std::vector<int32_t> segs;
-  };
-  std::vector<Bin> bins;
-  if (!segs.empty()) {
-    std::stable_sort(segs.begin(), segs.end(), [](const Seg& a, const Seg& b) { return a.lanes > b.lanes; });
-    std::set<std::pair<int32_t, int32_t>> open_any, open_final;  // (free lanes, bin)
-    for (int32_t i = 0; i < (int32_t)segs.size(); i++) {
-      const Seg& sg = segs[(size_t)i];
-      const bool multi = sg.nfull > 0;
-      int32_t b = -1;
-      const auto ia = open_any.lower_bound({sg.lanes, -1});
-      const auto jf = multi ? open_final.end() : open_final.lower_bound({sg.lanes, -1});
-      const bool has_a = ia != open_any.end(), has_f = jf != open_final.end();
-      if (has_f && (!has_a || jf->first < ia->first)) {
-        b = jf->second;
-        open_final.erase(jf);
-      } else if (has_a) {
-        b = ia->second;
-        open_any.erase(ia);
-      } else {
-        b = (int32_t)bins.size();
-        bins.emplace_back();
-      }
-      Bin& bn = bins[(size_t)b];
-      bn.used += sg.lanes;
-      bn.nmulti += multi ? 1 : 0;
-      bn.segs.push_back(i);
-      if (bn.used < 64) (bn.nmulti < 2 ? open_any : open_final).insert({64 - bn.used, b});
-    }
-    if (o.packing == 0) {
-      // keep the packing only if the filled bands still gain
-      double packed = 0, unpacked = 0;
-      static const int cand[7] = {8, 12, 16, 20, 25, 28, 32};
-      static const double f0[7] = {1.03, 1.011, 1.011, 1.0, 1.0, 1.0, 1.0};
-      for (const Seg& sg : segs) {
-        const int64_t q = lens[sg.read];
-        const int ci = (int)(std::find(cand, cand + 7, seg_of[(size_t)sg.read]) - cand);
-        const int64_t BR = 64 * cand[ci];
-        const int64_t nb = (q + BR - 1) / BR;
-        unpacked += (double)(nb * BR) * f0[ci] * (1.0 + 0.015 * (double)(nb - 1));
-        packed += (double)(sg.nfull * BRP) * (1.0 + 0.015 * sg.nfull);
-      }
-      packed += 1.02 * (double)BRP * (double)bins.size();
-      if (packed >= 0.99 * unpacked) {
-        segs.clear();
-        bins.clear();
-      }
-    }
-  }
+  std::vector<PackSeg> segs;
+  std::vector<PackBin> bins;
+  if (pack_form && (o.packing == 1 || (o.packing == 0 && !o.segment)))
+    plan_packing(lens, seg_of, SP, o.packing == 0, segs, bins);
   std::vector<char> packed_read((size_t)n_reads, 0);
-  for (const Seg& sg : segs) packed_read[(size_t)sg.read] = 1;
+  for (const PackSeg& sg : segs) packed_read[(size_t)sg.read] = 1;
 
   std::vector<int> widths;
   for (int32_t r = 0; r < n_reads; r++)
@@ -432,10 +433,10 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
 
   bool multiband = false;
   int32_t pack_rows = 1;  // band scratch rows per workgroup of the packed launch
-  for (const Bin& bn : bins) pack_rows = std::max(pack_rows, bn.nmulti);
+  for (const PackBin& bn : bins) pack_rows = std::max(pack_rows, bn.nmulti);
   int64_t slots = 1, slots_sum = 0;
   for (int32_t r = 0; r < n_reads; r++) multiband |= !packed_read[(size_t)r] && lens[r] > 64 * seg_of[(size_t)r];
-  for (const Seg& sg : segs) multiband |= sg.nfull > 0;
+  for (const PackSeg& sg : segs) multiband |= sg.nfull > 0;
   for (int S : widths) {
     const int64_t sl = (int64_t)std::max(1, dtwax::occupancy(p->precision, S, p->nstr, m3)) * c->n_cu;
     slots = std::max(slots, sl);
@@ -519,12 +520,12 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   if (!bins.empty()) {
     std::vector<std::pair<int64_t, WorkItem>> items;
     for (int32_t b = 0; b < (int32_t)bins.size(); b++) {
-      const Bin& bn = bins[(size_t)b];
+      const PackBin& bn = bins[(size_t)b];
       const int32_t first_read = (int32_t)hreads.size();
       int32_t lane = 0, min_lanes = 64, nmulti = 0;
       int64_t qb = 1;
       for (int32_t si : bn.segs) {
-        const Seg& sg = segs[(size_t)si];
+        const PackSeg& sg = segs[(size_t)si];
         const int32_t P = sg.lanes * SP - sg.rp;
         const int32_t endl = lane + sg.lanes - 1;
         for (int32_t l = 0; l < sg.lanes; l++) {
