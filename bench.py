@@ -393,8 +393,17 @@ def main():
             fp.run(d_norm, d_off, d_len, d_out, stream)
             fms.append(fp.kernel_ms())
         fi = fp.info()
+        # against the headline form on the same reads: score difference and
+        # decisions at the headline's threshold
+        fs = d_out.to_host()["score"]
+        rel = np.abs(fs - res["score"]) / np.maximum(np.abs(res["score"]), 1e-30)
+        flips = int(np.sum((fs.min(axis=1) < thr) != (best < thr)))
         forms[name] = {"kernel_ms": round(fms[-1], 3), "segment": fi["segment"],
-                       "kernel_gcups": round(fi["cells"] / (fms[-1] * 1e-3) / 1e9, 1)}
+                       "kernel_gcups": round(fi["cells"] / (fms[-1] * 1e-3) / 1e9, 1),
+                       "roofline_frac": round(3.0 * fi["cells"] / (fms[-1] * 1e-3) / peak_ops, 4),
+                       "vs_headline": {"max_rel_score_diff": float(f"{rel.max():.3g}"),
+                                       "mean_rel_score_diff": float(f"{rel.mean():.3g}"),
+                                       "decision_flips": flips}}
         fp.close()
     out["forms"] = forms
 
