@@ -23,6 +23,12 @@ actions it receives:
 
 `summary()` reports what sequencing yielded per class (reads, unblocked,
 samples sequenced), from which enrichment follows.
+
+Playback: with `playback` (a POD5 path or `Pod5File`), each new read is the
+next read of the file, in table order and cycling, sent as calibrated
+picoamperes ((raw + calibration_offset) x calibration_scale) under its own
+read id and at its own length; `playback_truth` ({read_id: is_target})
+labels them for `summary()` (unlisted reads count as non-target).
 """
 from __future__ import annotations
 
@@ -56,7 +62,7 @@ class SimulatedSequencer:
 
     def __init__(self, seq, model_means, n_channels=512, sample_rate=4000.0, read_mean=20000.0,
                  gap_s=1.0, target_frac=0.1, speed=1.0, interval_s=0.1, duration_s=None,
-                 min_read=1600, seed=0, k=6):
+                 min_read=1600, seed=0, k=6, playback=None, playback_truth=None):
         self.means = np.asarray(model_means, np.float32)
         self.k = k
         self.strands = (synth.kmer_codes(seq, k), synth.kmer_codes(synth.revcomp(seq), k))
@@ -70,9 +76,35 @@ class SimulatedSequencer:
         self.finished = []  # ended reads
         self.reads = {}  # (channel, number) -> every read started
         self.n_streams = 0
+        self._pb = None
+        if playback is not None:
+            from . import Pod5File
+            self._pb = playback if isinstance(playback, Pod5File) else Pod5File(playback)
+            self._pb_info = self._pb.reads()
+            self._pb_order = [i for i in range(self._pb.n_reads) if self._pb_info["n_samples"][i] > 0]
+            if not self._pb_order:
+                raise ValueError("the POD5 file holds no read with samples")
+            self._pb_next = 0
+            self._pb_truth = dict(playback_truth or {})
 
     # ---- flow cell ---------------------------------------------------------
+    def _playback_read(self, number, start):
+        i = self._pb_order[self._pb_next % len(self._pb_order)]
+        self._pb_next += 1
+        inf = self._pb_info[i]
+        x = self._pb.signal(reads=[i])[0].astype(np.float32)
+        off, sc = inf["calibration_offset"], inf["calibration_scale"]
+        if np.isfinite(off) and np.isfinite(sc):
+            x = (x + np.float32(off)) * np.float32(sc)
+        rid = inf["read_id"].decode()
+        r = _Read(number, start, x.size, bool(self._pb_truth.get(rid, False)), None)
+        r.id = rid
+        r.pending = x
+        return r
+
     def _new_read(self, number, start):
+        if self._pb is not None:
+            return self._playback_read(number, start)
         length = max(self.min_read, int(self.rng.exponential(self.read_mean)))
         target = bool(self.rng.random() < self.target_frac)
         if target:
@@ -85,8 +117,9 @@ class SimulatedSequencer:
         return _Read(number, start, length, target, codes)
 
     def _signal(self, r, n):
-        """The read's next n samples (continuing its k-mer walk)."""
-        while r.pending.size < n:
+        """The read's next n samples (continuing its k-mer walk, or the
+        played-back signal)."""
+        while self._pb is None and r.pending.size < n:
             block = max(n - r.pending.size, 4000)
             span = int(block / 9.0 * 1.5) + 64
             if r.codes is None:
@@ -217,3 +250,45 @@ class SimulatedSequencer:
         """{(channel, number): is_target} of every read started."""
         with self.lock:
             return {key: r.target for key, r in self.reads.items()}
+
+
+def main(argv=None):
+    """dtwax-sim-sequencer: serve a simulated flow cell (or a POD5 playback)
+    over MinKNOW's data service for dry runs of `python -m dtwax_amd.minknow`."""
+    import argparse
+    from concurrent import futures
+
+    import grpc
+
+    import dtwax_amd as D
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("model", help="k-mer model (KMER mean sd)")
+    ap.add_argument("fasta", help="reference FASTA: targets are drawn from it")
+    ap.add_argument("--port", type=int, default=8000)
+    ap.add_argument("--channels", type=int, default=512)
+    ap.add_argument("--speed", type=float, default=1.0, help="simulated seconds per wall second")
+    ap.add_argument("--target-frac", type=float, default=0.1)
+    ap.add_argument("--read-mean", type=float, default=20000.0, help="mean read length (samples)")
+    ap.add_argument("--playback", help="POD5 file whose reads are served instead of simulated ones")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args(argv)
+    means = D.read_kmer_model(a.model)
+    k = int(round(np.log(means.size) / np.log(4)))
+    sim = SimulatedSequencer(D.read_fasta(a.fasta), means, n_channels=a.channels, read_mean=a.read_mean,
+                             target_frac=a.target_frac, speed=a.speed, seed=a.seed, k=k,
+                             playback=a.playback)
+    server = grpc.server(futures.ThreadPoolExecutor(max_workers=8))
+    mk.add_data_service(server, sim)
+    server.add_insecure_port(f"127.0.0.1:{a.port}")
+    server.start()
+    print(f"serving {a.channels} channels on 127.0.0.1:{a.port}", flush=True)
+    try:
+        server.wait_for_termination()
+    except KeyboardInterrupt:
+        server.stop(0)
+    print(sim.summary())
+
+
+if __name__ == "__main__":
+    main()

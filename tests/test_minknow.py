@@ -306,3 +306,49 @@ def test_setup_errors(sequencer):
     with pytest.raises(grpc.RpcError) as e:
         next(call)
     assert e.value.code() == grpc.StatusCode.INVALID_ARGUMENT
+
+
+def test_playback_of_a_pod5_file(tmp_path):
+    """The simulated sequencer replays a POD5 file: each read arrives under
+    its POD5 read id, as contiguous chunks of its calibrated signal, and the
+    truth labels come from playback_truth."""
+    import threading
+    from dtwax_amd import minknow_sim, synth
+    from pod5_helpers import make_read, write_pod5
+    rng = np.random.default_rng(8)
+    reads = [make_read(rng.integers(-500, 1500, n).astype(np.int16), i) for i, n in
+             enumerate([9000, 0, 4000, 12000])]
+    write_pod5(tmp_path / "run.pod5", reads, row_samples=2500)
+    means, _ = synth.make_model(rng)
+    truth = {reads[0]["read_id"]: True}
+    sim = minknow_sim.SimulatedSequencer(synth.random_seq(rng, 2000), means, n_channels=3, gap_s=0.05,
+                                         speed=40.0, interval_s=0.01, duration_s=6.0, seed=1,
+                                         playback=str(tmp_path / "run.pod5"), playback_truth=truth)
+    P = mk.protocol()
+    R = P.GetLiveReadsRequest
+    done = threading.Event()
+
+    def requests():
+        yield R(setup=R.StreamSetup(first_channel=1, last_channel=3,
+                                    raw_data_type=P.RAW_DATA_TYPES["calibrated"]))
+        done.wait(30)
+
+    ctx = SimpleNamespace(is_active=lambda: True, abort=None)
+    got = {}  # (channel, read number) -> (read id, chunks); the file's reads cycle
+    for msg in sim.get_live_reads(requests(), ctx):
+        for ch, rd in msg.channels.items():
+            rid, parts = got.setdefault((ch, rd.number), (rd.id, []))
+            assert rd.id == rid
+            assert rd.chunk_start_sample == rd.start_sample + sum(p.size for p in parts)
+            parts.append(np.frombuffer(rd.raw_data, "<f4"))
+    done.set()
+    want = {r["read_id"]: (r["signal"].astype(np.float32) + np.float32(r["calibration_offset"]))
+            * np.float32(r["calibration_scale"]) for r in reads if r["signal"].size}
+    assert len(got) >= 3 and {rid for rid, _ in got.values()} <= set(want)
+    for rid, parts in got.values():
+        x = np.concatenate(parts)
+        assert x.size <= want[rid].size
+        assert np.array_equal(x, want[rid][:x.size])
+    s = sim.summary()
+    assert s["target"]["reads"] + s["nontarget"]["reads"] >= 2
+    assert all(r.target == (r.id == reads[0]["read_id"]) for r in sim.reads.values())
