@@ -49,6 +49,7 @@ namespace dtwax {
 int set_error(int code, const std::string& msg);
 bool zstd_frame(const uint8_t* src, size_t n, size_t max_out, std::vector<uint8_t>& out, std::string& err);
 bool svb16_decode(const uint8_t* blk, size_t size, int64_t n, int16_t* dst);
+bool svb16_zstd_prefix(const uint8_t* src, size_t n_src, int64_t n, int64_t m, int16_t* dst, std::string& err);
 }  // namespace dtwax
 
 namespace {
@@ -509,14 +510,23 @@ struct dtwax_pod5 {
     return (uint32_t)sig[b].samples.iat(*r);
   }
 
-  // Decodes one signal row (its `samples` values) into dst.
-  bool decode_row(uint64_t row, int16_t* dst, std::vector<uint8_t>& tmp, std::string& err) const {
+  // Decodes the first m values of one signal row (all `samples` of them
+  // when m is -1) into dst.
+  bool decode_row(uint64_t row, int16_t* dst, std::vector<uint8_t>& tmp, std::string& err, int64_t m = -1) const {
     const SigBatch* sb;
     int64_t r;
     const uint32_t n = row_samples(row, &sb, &r);
     const int64_t a = sb->sig.begin(r), z = sb->sig.begin(r + 1);
+    if (m < 0 || m > (int64_t)n) m = n;
     if (!sb->vbz) {
-      std::memcpy(dst, sb->sig.values.p + 2 * (size_t)a, 2 * (size_t)n);  // length checked at open
+      std::memcpy(dst, sb->sig.values.p + 2 * (size_t)a, 2 * (size_t)m);  // length checked at open
+      return true;
+    }
+    if (m < (int64_t)n) {  // a prefix: only the keys and the leading data are decompressed
+      if (!dtwax::svb16_zstd_prefix(sb->sig.values.p + a, (size_t)(z - a), n, m, dst, err)) {
+        err = "signal row " + std::to_string(row) + ": " + err;
+        return false;
+      }
       return true;
     }
     const size_t max_out = ((size_t)n + 7) / 8 + 2 * (size_t)n;
@@ -752,7 +762,8 @@ extern "C" int dtwax_pod5_signal(const dtwax_pod5* f, const int64_t* reads, int6
             row.resize((size_t)c);
             into = row.data();
           }
-          if (!f->decode_row(f->rows[(size_t)k], into, tmp, err)) break;
+          // a row the window ends in is decoded only up to the window's end
+          if (!f->decode_row(f->rows[(size_t)k], into, tmp, err, z - pos)) break;
           if (into == row.data()) std::memcpy(out + (a - skip), row.data() + (a - pos), 2 * (size_t)(z - a));
           wrote = z - skip;
         }

@@ -41,6 +41,16 @@ int set_error(int code, const std::string& msg);
 
 namespace {
 
+// zstd's streaming buffers (zstd.h, stable API)
+struct ZInBuf {
+  const void* src;
+  size_t size, pos;
+};
+struct ZOutBuf {
+  void* dst;
+  size_t size, pos;
+};
+
 struct Zstd {
   bool ok = false;
   std::string err;
@@ -48,6 +58,11 @@ struct Zstd {
   unsigned long long (*frame_size)(const void*, size_t) = nullptr;
   unsigned (*is_error)(size_t) = nullptr;
   const char* (*error_name)(size_t) = nullptr;
+  // streaming (optional: without it a prefix is cut from the whole frame)
+  void* (*create_dctx)() = nullptr;
+  size_t (*free_dctx)(void*) = nullptr;
+  size_t (*dctx_reset)(void*, int) = nullptr;
+  size_t (*decompress_stream)(void*, ZOutBuf*, ZInBuf*) = nullptr;
 };
 
 Zstd& zstd() {
@@ -66,6 +81,10 @@ Zstd& zstd() {
     a.frame_size = reinterpret_cast<decltype(a.frame_size)>(dlsym(h, "ZSTD_getFrameContentSize"));
     a.is_error = reinterpret_cast<decltype(a.is_error)>(dlsym(h, "ZSTD_isError"));
     a.error_name = reinterpret_cast<decltype(a.error_name)>(dlsym(h, "ZSTD_getErrorName"));
+    a.create_dctx = reinterpret_cast<decltype(a.create_dctx)>(dlsym(h, "ZSTD_createDCtx"));
+    a.free_dctx = reinterpret_cast<decltype(a.free_dctx)>(dlsym(h, "ZSTD_freeDCtx"));
+    a.dctx_reset = reinterpret_cast<decltype(a.dctx_reset)>(dlsym(h, "ZSTD_DCtx_reset"));
+    a.decompress_stream = reinterpret_cast<decltype(a.decompress_stream)>(dlsym(h, "ZSTD_decompressStream"));
     if (!a.decompress || !a.frame_size || !a.is_error || !a.error_name) {
       a.err = "zstd library lacks required symbols";
       return a;
@@ -126,6 +145,50 @@ bool zstd_frame(const uint8_t* src, size_t n, size_t max_out, std::vector<uint8_
   return true;
 }
 
+// The first `want` bytes of one zstd frame (fewer when the frame is
+// shorter), decompressed in two steps so that the caller can size the
+// second from the first: out receives [0, want) after each call, the
+// frame's state kept in between (one object per thread).
+class ZstdPrefix {
+ public:
+  ~ZstdPrefix() {
+    if (dctx_) zstd().free_dctx(dctx_);
+  }
+  bool start(const uint8_t* src, size_t n, std::string& err) {
+    Zstd& z = zstd();
+    if (!z.ok) return err = z.err, false;
+    if (!z.create_dctx || !z.dctx_reset || !z.decompress_stream) return err = "zstd lacks streaming", false;
+    if (!dctx_ && !(dctx_ = z.create_dctx())) return err = "zstd context allocation failed", false;
+    z.dctx_reset(dctx_, 1);  // ZSTD_reset_session_only
+    in_ = {src, n, 0};
+    done_ = false;
+    return true;
+  }
+  bool fill(std::vector<uint8_t>& out, size_t want, std::string& err) {
+    Zstd& z = zstd();
+    const size_t have = out.size();
+    if (want <= have || done_) return true;
+    out.resize(want);
+    ZOutBuf ob{out.data(), want, have};
+    while (ob.pos < want) {
+      const size_t r = z.decompress_stream(dctx_, &ob, &in_);
+      if (z.is_error(r)) return err = std::string("zstd: ") + z.error_name(r), false;
+      if (r == 0) {  // frame complete
+        done_ = true;
+        break;
+      }
+      if (in_.pos == in_.size && ob.pos < want) return err = "zstd frame truncated", false;
+    }
+    out.resize(ob.pos);
+    return true;
+  }
+
+ private:
+  void* dctx_ = nullptr;
+  ZInBuf in_{nullptr, 0, 0};
+  bool done_ = false;
+};
+
 // 1-bit-key stream-VByte block of n zig-zag 16-bit differences (the first
 // against 0), which must account for every byte of the block.
 bool svb16_decode(const uint8_t* blk, size_t size, int64_t n, int16_t* dst) {
@@ -135,6 +198,46 @@ bool svb16_decode(const uint8_t* blk, size_t size, int64_t n, int16_t* dst) {
   for (int64_t i = 0; i < n; i++) {
     uint16_t u = p[0];
     if ((blk[i >> 3] >> (i & 7)) & 1) {
+      u = (uint16_t)(u | (uint16_t)p[1] << 8);
+      p += 2;
+    } else {
+      p += 1;
+    }
+    prev = (uint16_t)(prev + (uint16_t)((u >> 1) ^ (uint16_t)(0u - (u & 1u))));
+    dst[i] = (int16_t)prev;
+  }
+  return true;
+}
+
+// The first m of the n values of a 1-bit-key stream-VByte block of zig-zag
+// 16-bit differences, from the zstd frame `src`: only the keys and the
+// leading data bytes are decompressed.  Checks the keys and bytes it reads
+// (not the rest of the block).
+bool svb16_zstd_prefix(const uint8_t* src, size_t n_src, int64_t n, int64_t m, int16_t* dst, std::string& err) {
+  thread_local ZstdPrefix zp;
+  thread_local std::vector<uint8_t> buf;
+  buf.clear();
+  if (!zstd().decompress_stream) {  // no streaming API: the whole frame
+    const size_t max_out = (size_t)(n + 7) / 8 + 2 * (size_t)n;
+    std::vector<int16_t> all((size_t)n);
+    if (!zstd_frame(src, n_src, max_out, buf, err)) return false;
+    if (!svb16_decode(buf.data(), buf.size(), n, all.data()))
+      return err = "VBZ block does not hold " + std::to_string(n) + " samples", false;
+    std::memcpy(dst, all.data(), 2 * (size_t)m);
+    return true;
+  }
+  const size_t nkey = (size_t)(n + 7) / 8;
+  if (!zp.start(src, n_src, err) || !zp.fill(buf, nkey, err)) return false;
+  if (buf.size() < nkey) return err = "VBZ block shorter than its keys", false;
+  size_t need = nkey;
+  for (int64_t i = 0; i < m; i++) need += 1 + ((buf[(size_t)(i >> 3)] >> (i & 7)) & 1);
+  if (!zp.fill(buf, need, err)) return false;
+  if (buf.size() < need) return err = "VBZ block shorter than its keys announce", false;
+  const uint8_t* p = buf.data() + nkey;
+  uint16_t prev = 0;
+  for (int64_t i = 0; i < m; i++) {
+    uint16_t u = p[0];
+    if ((buf[(size_t)(i >> 3)] >> (i & 7)) & 1) {
       u = (uint16_t)(u | (uint16_t)p[1] << 8);
       p += 2;
     } else {
