@@ -56,7 +56,7 @@ std::string tune_key(const dtwax_ctx* c, const dtwax_score_opts& o, const std::v
   const size_t n = sorted_lens.size();
   auto q = [&](double f) { return sorted_lens[std::min(n - 1, (size_t)(f * (double)(n - 1)))] / 64; };
   std::ostringstream k;
-  k << c->name << "/cu" << c->n_cu << "/p" << o.precision << "/S" << o.segment << "/C" << o.ref_chunk << "/m" << (o.moves == 3 ? 3 : 2) << "/s"
+  k << c->name << "/cu" << c->n_cu << "/p" << o.precision << "/S" << o.segment << "/C" << o.ref_chunk << "/m" << (o.moves == 3 ? 3 : 2) << "/k" << o.packing << "/s"
     << c->n_strands << "/N" << bucket_log2(c->n_cols)
     << "/n" << bucket_log2((int64_t)n) << "/q" << q(0.1) << "," << q(0.5) << "," << q(0.9);
   return k.str();
