@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--reads", type=int, default=4096)
     ap.add_argument("--precision", default="f16")
     ap.add_argument("--segment", type=int, default=0)
+    ap.add_argument("--packing", type=int, default=0, choices=(-1, 0, 1),
+                    help="plan packing: 0 automatic, 1 always, -1 never")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the bounded cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -190,7 +192,7 @@ def main_c4(a):
                                    cache_path=a.tune_cache, precision=a.precision)
     tune_s = time.time() - t1
     print(f"[bench] c4 autotune {best} in {tune_s:.1f} s", file=sys.stderr, flush=True)
-    plan = ctx.plan(lens, precision=a.precision, **best)
+    plan = ctx.plan(lens, precision=a.precision, packing=a.packing, **best)
     pinfo = plan.info()
     plan.set_timing(True)
     for _ in range(warmup):
@@ -283,7 +285,7 @@ def main():
     d_len = D.DeviceArray.from_host(lens)
     d_norm = D.DeviceArray(flat.shape, np.float32)
     d_out = D.DeviceArray((len(raw),), D.RESULT_DTYPE)
-    plan = ctx.plan(lens, precision=a.precision, segment=a.segment)
+    plan = ctx.plan(lens, precision=a.precision, segment=a.segment, packing=a.packing)
     pinfo = plan.info()
     plan.set_timing(True)
     L = D.lib()
@@ -362,7 +364,7 @@ def main():
                                "both strands, 50% target, median/MAD",
                    "reads": len(raw), "query_len": int(lens[0]), "ref_cols_per_strand": int(n_cols),
                    "segment": pinfo["segment"], "work_items": pinfo["n_items"],
-                   "grid": pinfo["grid"]},
+                   "grid": pinfo["grid"], "packed_bands": pinfo["packs"], "packed_reads": pinfo["packed_reads"]},
         "reads_classified_per_s": round(len(raw) / (step_ms * 1e-3), 1),
         "kernel_ms": round(kms, 3),
         "kernel_gcups": round(cell_rate / 1e9, 1),

@@ -295,11 +295,11 @@ struct PackBin {
 // A read needs at least 4 lanes (the history flush period is at least 4
 // steps).  Automatic: the reads whose packed cost beats their unpacked width
 // by 1%, kept only if the bands as filled still beat those reads' unpacked
-// cost.  Placement: best fit decreasing, at most 2 continuing reads per band
-// (each needs a band scratch row per wave; a third fills the bands no better
-// on uneven batches and would halve the chunk the scratch budget allows).
-void plan_packing(const std::vector<int32_t>& lens, const std::vector<int>& seg_of, int SP, bool automatic,
-                  std::vector<PackSeg>& segs, std::vector<PackBin>& bins) {
+// cost.  Placement: best fit decreasing, at most max_multi continuing reads
+// per band (each needs a band scratch row per wave).  Returns the packed
+// reads' cost (band_cost units), 0 when nothing is packed.
+double plan_packing(const std::vector<int32_t>& lens, const std::vector<int>& seg_of, int SP, bool automatic,
+                    int max_multi, std::vector<PackSeg>& segs, std::vector<PackBin>& bins) {
   const int64_t BRP = 64 * (int64_t)SP;
   for (int32_t r = 0; r < (int32_t)lens.size(); r++) {
     const int64_t q = lens[(size_t)r];
@@ -311,7 +311,7 @@ void plan_packing(const std::vector<int32_t>& lens, const std::vector<int>& seg_
     if (automatic && pk >= 0.99 * band_cost(q, seg_of[(size_t)r])) continue;
     segs.push_back({r, lanes, nfull, rp});
   }
-  if (segs.empty()) return;
+  if (segs.empty()) return 0;
   std::stable_sort(segs.begin(), segs.end(), [](const PackSeg& a, const PackSeg& b) { return a.lanes > b.lanes; });
   std::set<std::pair<int32_t, int32_t>> open_any, open_final;  // (free lanes, bin)
   for (int32_t i = 0; i < (int32_t)segs.size(); i++) {
@@ -319,7 +319,7 @@ void plan_packing(const std::vector<int32_t>& lens, const std::vector<int>& seg_
     const bool multi = sg.nfull > 0;
     int32_t b = -1;
     const auto ia = open_any.lower_bound({sg.lanes, -1});
-    const auto jf = multi ? open_final.end() : open_final.lower_bound({sg.lanes, -1});
+    const auto jf = multi ? open_final.end() : open_final.lower_bound({sg.lanes, -1});  // full of continuing reads
     const bool has_a = ia != open_any.end(), has_f = jf != open_final.end();
     if (has_f && (!has_a || jf->first < ia->first)) {
       b = jf->second;
@@ -335,19 +335,19 @@ void plan_packing(const std::vector<int32_t>& lens, const std::vector<int>& seg_
     bn.used += sg.lanes;
     bn.nmulti += multi ? 1 : 0;
     bn.segs.push_back(i);
-    if (bn.used < 64) (bn.nmulti < 2 ? open_any : open_final).insert({64 - bn.used, b});
+    if (bn.used < 64) (bn.nmulti < max_multi ? open_any : open_final).insert({64 - bn.used, b});
   }
-  if (automatic) {
-    double packed = kPackCost * (double)BRP * (double)bins.size(), unpacked = 0;
-    for (const PackSeg& sg : segs) {
-      unpacked += band_cost(lens[(size_t)sg.read], seg_of[(size_t)sg.read]);
-      packed += (double)(sg.nfull * BRP) * (1.0 + 0.015 * sg.nfull);
-    }
-    if (packed >= 0.99 * unpacked) {
-      segs.clear();
-      bins.clear();
-    }
+  double packed = kPackCost * (double)BRP * (double)bins.size(), unpacked = 0;
+  for (const PackSeg& sg : segs) {
+    unpacked += band_cost(lens[(size_t)sg.read], seg_of[(size_t)sg.read]);
+    packed += (double)(sg.nfull * BRP) * (1.0 + 0.015 * sg.nfull);
   }
+  if (automatic && packed >= 0.99 * unpacked) {
+    segs.clear();
+    bins.clear();
+    return 0;
+  }
+  return packed;
 }
 }  // namespace
 
@@ -407,14 +407,46 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   }
   if (o.packing < -1 || o.packing > 1) return fail(DTWAX_EINVAL, "packing must be -1, 0 or 1");
   const bool pack_form = !m3 && (o.precision == DTWAX_F16 || o.precision == DTWAX_F16_FUSED);
-  if (o.packing == 1 && (!pack_form || (o.segment && o.segment != 16 && o.segment != 32)))
-    return fail(DTWAX_EINVAL, "packing needs DTWAX_F16 or DTWAX_F16_FUSED, two moves and segment 0, 16 or 32");
-  const int SP = (o.segment == 16 || o.segment == 32) ? o.segment : 32;
-  const int64_t BRP = 64 * SP;
+  if (o.packing == 1 && (!pack_form || (o.segment && o.segment < 16)))
+    return fail(DTWAX_EINVAL, "packing needs DTWAX_F16 or DTWAX_F16_FUSED, two moves and segment 0 or 16..32");
+  // Packing width: the segment when given, else the width whose packing
+  // (with the other reads at their own widths) costs least.  Up to 4
+  // continuing reads per band when whole-strand rows for them fit the
+  // scratch budget, else 2 (a chunk shortened for the rows costs more
+  // overlap than the extra reads save).
+  int SP = o.segment ? o.segment : 32;
   std::vector<PackSeg> segs;
   std::vector<PackBin> bins;
-  if (pack_form && (o.packing == 1 || (o.packing == 0 && !o.segment)))
-    plan_packing(lens, seg_of, SP, o.packing == 0, segs, bins);
+  if (pack_form && (o.packing == 1 || (o.packing == 0 && !o.segment))) {
+    auto max_multi = [&](int S) {
+      const int64_t slots_p = (int64_t)std::max(1, occupancy_f16_pack(o.precision, S)) * c->n_cu;
+      const int64_t row = (COL_PADL + N + qmax + COL_PADR) * 4;
+      return row * 4 * slots_p <= scratch_budget() ? 4 : 2;
+    };
+    if (o.packing == 1) {
+      plan_packing(lens, seg_of, SP, false, max_multi(SP), segs, bins);
+    } else {
+      double best = 0;
+      for (int32_t r = 0; r < n_reads; r++) best += band_cost(lens[r], seg_of[(size_t)r]);
+      for (int S : {32, 28, 25, 20, 16}) {
+        std::vector<PackSeg> sg;
+        std::vector<PackBin> bn;
+        double cost = plan_packing(lens, seg_of, S, true, max_multi(S), sg, bn);
+        if (sg.empty()) continue;
+        std::vector<char> in((size_t)n_reads, 0);
+        for (const PackSeg& x : sg) in[(size_t)x.read] = 1;
+        for (int32_t r = 0; r < n_reads; r++)
+          if (!in[(size_t)r]) cost += band_cost(lens[r], seg_of[(size_t)r]);
+        if (cost < 0.995 * best) {
+          best = cost;
+          SP = S;
+          segs.swap(sg);
+          bins.swap(bn);
+        }
+      }
+    }
+  }
+  const int64_t BRP = 64 * SP;
   std::vector<char> packed_read((size_t)n_reads, 0);
   for (const PackSeg& sg : segs) packed_read[(size_t)sg.read] = 1;
 

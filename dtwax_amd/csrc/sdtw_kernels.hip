@@ -1111,9 +1111,14 @@ static hipError_t launch_f16_s(const KParams& p, int S, int grid, bool m3, hipSt
 
 template <int MODE>
 static hipError_t launch_f16_pack(const KParams& p, int S, int grid, hipStream_t st) {
-  if (S == 32) hipLaunchKernelGGL((sdtw_f16_pack_kernel<32, MODE>), dim3(grid), dim3(64), 0, st, p);
-  else if (S == 16) hipLaunchKernelGGL((sdtw_f16_pack_kernel<16, MODE>), dim3(grid), dim3(64), 0, st, p);
-  else return hipErrorInvalidValue;
+  switch (S) {
+    case 16: hipLaunchKernelGGL((sdtw_f16_pack_kernel<16, MODE>), dim3(grid), dim3(64), 0, st, p); break;
+    case 20: hipLaunchKernelGGL((sdtw_f16_pack_kernel<20, MODE>), dim3(grid), dim3(64), 0, st, p); break;
+    case 25: hipLaunchKernelGGL((sdtw_f16_pack_kernel<25, MODE>), dim3(grid), dim3(64), 0, st, p); break;
+    case 28: hipLaunchKernelGGL((sdtw_f16_pack_kernel<28, MODE>), dim3(grid), dim3(64), 0, st, p); break;
+    case 32: hipLaunchKernelGGL((sdtw_f16_pack_kernel<32, MODE>), dim3(grid), dim3(64), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -1201,15 +1206,20 @@ int occupancy_f16(int precision, int S, bool m3) {
 
 int occupancy_f16_pack(int precision, int S) {
   int n = 0;
-  if (precision == DTWAX_F16) {
-    if (S == 32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<32, DTWAX_F16>, 64, 0);
-    if (S == 16) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<16, DTWAX_F16>, 64, 0);
-  } else if (precision == DTWAX_F16_FUSED) {
-    if (S == 32)
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<32, DTWAX_F16_FUSED>, 64, 0);
-    if (S == 16)
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<16, DTWAX_F16_FUSED>, 64, 0);
+#define OCCP(MODE, SS)                                                                          \
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_pack_kernel<SS, MODE>, 64, 0);
+#define OCCP_S(MODE)                 \
+  switch (S) {                       \
+    case 16: OCCP(MODE, 16) break;   \
+    case 20: OCCP(MODE, 20) break;   \
+    case 25: OCCP(MODE, 25) break;   \
+    case 28: OCCP(MODE, 28) break;   \
+    case 32: OCCP(MODE, 32) break;   \
   }
+  if (precision == DTWAX_F16) OCCP_S(DTWAX_F16)
+  else if (precision == DTWAX_F16_FUSED) OCCP_S(DTWAX_F16_FUSED)
+#undef OCCP_S
+#undef OCCP
   return n;
 }
 

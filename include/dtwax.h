@@ -103,8 +103,10 @@ typedef struct {
                             the reads that gain), -1 = never.  A packed read's rows start
                             with a partial band (its length modulo 64 x S rows, padded to a
                             lane) that shares one band with other reads' partial bands,
-                            followed by its full bands; S = 32, or the segment when 16 or
-                            32.  Scores are identical to the unpacked layout's */
+                            followed by its full bands.  S: the segment when given (16 to
+                            32), else 32 with packing = 1, or the width of 16, 20, 25, 28
+                            and 32 whose packing costs least.  Scores are identical to the
+                            unpacked layout's */
 } dtwax_score_opts;
 
 /* ---- context ----------------------------------------------------------- */

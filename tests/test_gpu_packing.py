@@ -76,6 +76,21 @@ def test_packed_width16_and_scratch_rows(ctx):
     assert np.array_equal(sc, osc) and np.array_equal(am, oam)
 
 
+@pytest.mark.parametrize("segment,precision", [(20, "f16"), (25, "f16"), (28, "f16_fused"), (25, "f16_fused")])
+def test_packed_other_widths(ctx, segment, precision):
+    """Packing at S = 20, 25 and 28: 2000-sample reads at S = 25 are one full
+    band and a 16-lane partial band, four partials to a shared band."""
+    rng = np.random.default_rng(segment)
+    ref = rng.standard_normal((2, 1300)).astype(np.float32)
+    ctx.upload_reference(ref)
+    lens = [2000] * 6 + [300, 777, 1280, 1281, 2600, 4100, 150]
+    reads = [rng.standard_normal(n).astype(np.float32) for n in lens]
+    sc, am, info = _run(ctx, reads, precision=precision, packing=1, segment=segment, ref_chunk=500)
+    assert info["packs"] > 0 and info["segment"] in (0, segment)
+    osc, oam = _oracle(reads, ref, fused=precision == "f16_fused")
+    assert np.array_equal(sc, osc) and np.array_equal(am, oam)
+
+
 def test_automatic_packing_uneven(ctx):
     """Automatic plans pack uneven reads that gain, leave a uniform batch
     unpacked, and every layout scores the same."""
