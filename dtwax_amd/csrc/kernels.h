@@ -89,7 +89,7 @@ struct KParams {
 };
 
 // m3: three moves (the left move D[r][c-1] too; DTWAX_F16 and DTWAX_F32 only)
-// pack: the packed kernel (DTWAX_F16 / DTWAX_F16_FUSED, S = 16 or 32, two moves)
+// pack: the packed kernel (DTWAX_F16 / DTWAX_F16_FUSED, S = 16, 20, 25, 28 or 32, two moves)
 hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool m3,
                        hipStream_t st, bool pack = false);
 // zero n_counters work counters (n_counters <= 256) and set n_keys keys to ~0
