@@ -376,6 +376,10 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     if (q < 1) return fail(DTWAX_EINVAL, "every read needs at least one sample");
     qmax = std::max(qmax, q);
   }
+  if (o.packing < -1 || o.packing > 1) return fail(DTWAX_EINVAL, "packing must be -1, 0 or 1");
+  const bool pack_form = !m3 && (o.precision == DTWAX_F16 || o.precision == DTWAX_F16_FUSED);
+  if (o.packing == 1 && (!pack_form || (o.segment && o.segment < 16)))
+    return fail(DTWAX_EINVAL, "packing needs DTWAX_F16 or DTWAX_F16_FUSED, two moves and segment 0 or 16..32");
   HIP_TRY(hipSetDevice(c->device));
 
   auto* p = new dtwax_plan;
@@ -405,10 +409,6 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   } else {
     for (int32_t r = 0; r < n_reads; r++) seg_of[(size_t)r] = best_width(lens[r]);
   }
-  if (o.packing < -1 || o.packing > 1) return fail(DTWAX_EINVAL, "packing must be -1, 0 or 1");
-  const bool pack_form = !m3 && (o.precision == DTWAX_F16 || o.precision == DTWAX_F16_FUSED);
-  if (o.packing == 1 && (!pack_form || (o.segment && o.segment < 16)))
-    return fail(DTWAX_EINVAL, "packing needs DTWAX_F16 or DTWAX_F16_FUSED, two moves and segment 0 or 16..32");
   // Packing width: the segment when given, else the width whose packing
   // (with the other reads at their own widths) costs least.  Up to 4
   // continuing reads per band when whole-strand rows for them fit the
@@ -661,14 +661,17 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     return cleanup("counter allocation failed");
   if (hipMalloc(&p->d_keys, std::max<size_t>(1, (size_t)n_reads * 2) * sizeof(unsigned long long)) != hipSuccess)
     return cleanup("key allocation failed");
-  if (!hv.empty()) HIP_TRY(hipMemcpy(p->d_items, hv.data(), hv.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+  if (!hv.empty() && hipMemcpy(p->d_items, hv.data(), hv.size() * sizeof(WorkItem), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup("item upload failed");
   if (!hlanes.empty()) {
     if (hipMalloc(&p->d_pack_lanes, hlanes.size() * sizeof(PackLane)) != hipSuccess ||
         hipMalloc(&p->d_pack_reads, std::max<size_t>(1, hreads.size()) * sizeof(PackRead)) != hipSuccess)
       return cleanup("pack table allocation failed");
-    HIP_TRY(hipMemcpy(p->d_pack_lanes, hlanes.data(), hlanes.size() * sizeof(PackLane), hipMemcpyHostToDevice));
-    if (!hreads.empty())
-      HIP_TRY(hipMemcpy(p->d_pack_reads, hreads.data(), hreads.size() * sizeof(PackRead), hipMemcpyHostToDevice));
+    if (hipMemcpy(p->d_pack_lanes, hlanes.data(), hlanes.size() * sizeof(PackLane), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        (!hreads.empty() && hipMemcpy(p->d_pack_reads, hreads.data(), hreads.size() * sizeof(PackRead),
+                                      hipMemcpyHostToDevice) != hipSuccess))
+      return cleanup("pack table upload failed");
   }
   if (p->groups.size() > 1) {
     if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess)
@@ -685,7 +688,7 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     p->scratch_stride = COL_PADL + max_w + COL_PADR;
     const size_t bytes = (size_t)scratch_rows * (size_t)p->scratch_stride * (size_t)epc * 4;
     if (hipMalloc(&p->d_scratch, bytes) != hipSuccess) return cleanup("band scratch allocation failed");
-    HIP_TRY(hipMemset(p->d_scratch, 0, bytes));
+    if (hipMemset(p->d_scratch, 0, bytes) != hipSuccess) return cleanup("band scratch clear failed");
   }
   *out = p;
   return DTWAX_OK;
