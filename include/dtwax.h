@@ -20,6 +20,9 @@
  *   dtwax_vbz_decode       the VBZ HDF5 plugin's decode   fast5/nanopolish_fast5_io.cpp:162-176
  *                                                         (vbz_compression v1.0.1,
  *                                                         install_hdf5_vbz.sh:43-58)
+ *   dtwax_pod5_*           no counterpart: POD5 input beside the FAST5 reader
+ *                                                         (fast5/nanopolish_fast5_io.cpp:72-194)
+ *   dtwax_plan_packing     no counterpart: reports this library's packed plan layout
  */
 #ifndef DTWAX_H
 #define DTWAX_H
@@ -244,8 +247,9 @@ int dtwax_score_plan(dtwax_plan* plan, const float* d_queries, const int64_t* d_
 /* Enable HIP-event timing of the sDTW kernel launches inside
  * dtwax_score_plan; dtwax_plan_kernel_ms then waits for and returns the last
  * call's sDTW time (all of its launches). */
-/* Packed layout of a plan: bands shared by several reads' partial bands,
- * and the reads placed in them (0, 0 when the plan packs nothing). */
+/* Packed layout of a plan (no counterpart in the reference, whose batches
+ * give every read whole bands): bands shared by several reads' partial
+ * bands, and the reads placed in them (0, 0 when the plan packs nothing). */
 int dtwax_plan_packing(const dtwax_plan* plan, int32_t* n_packs, int32_t* n_packed_reads);
 int dtwax_plan_set_timing(dtwax_plan* plan, int32_t enable);
 int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
