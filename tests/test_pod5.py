@@ -328,3 +328,22 @@ def test_cli_scores_pod5_like_fast5(tmp_path):
 
     a, b = table(d), table(GOLDEN)
     assert a and a == b
+
+
+def test_cli_mixed_fast5_and_pod5_folder(tmp_path):
+    """A folder with a FAST5 and a POD5 file: the reads of both are loaded
+    (each file's reads in its own order), short ones exempted in both."""
+    import shutil
+    sig = np.load(os.path.join(GOLDEN, "batch0_signal.npy")).astype(np.int16)
+    d = tmp_path / "mixed"
+    d.mkdir()
+    shutil.copy(os.path.join(GOLDEN, "batch0.fast5"), d / "batch0.fast5")
+    write_pod5(d / "more.pod5", [make_read(sig[::-1].copy(), 1), make_read(sig[:1000], 2)])
+    lines, _ = _dump(d)
+    assert "cuDTW:: 2 ONT fast5 files read" in lines
+    assert "cuDTW:: Short/invalid reads exempted:: 1" in lines
+    assert "cuDTW:: Valid reads (NUM_READS) loaded :: 2" in lines
+    rows = {l.split()[0]: np.array(l.split()[1:], np.float64) for l in lines if l.startswith("read_")}
+    assert np.array_equal(rows["read_0083a4a2-b062-44ed-988e-e7f5a301121f"], sig[1952:4000].astype(np.float64))
+    rid = "read_" + make_read(sig, 1)["read_id"]
+    assert np.array_equal(rows[rid], sig[::-1][1952:4000].astype(np.float64))
