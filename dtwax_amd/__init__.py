@@ -403,6 +403,34 @@ class Pod5File:
         return dst[: idx.size, :take], n_out[: idx.size]
 
 
+POD5_RESULT_DTYPE = np.dtype([("read_id", "U36"), ("score", np.float32, 2), ("argmin", np.int32, 2),
+                              ("target", np.bool_)])
+
+
+def classify_pod5(ctx, path, skip=1952, take=2048, norm=NORM_MEANSTD, eps=1e-4, threshold=None,
+                  threads=0, **score_kw):
+    """Scores a POD5 file's reads as dtwax_main scores its input: samples
+    [skip, skip + take) of every read that has that many (shorter reads are
+    left out), normalised on the GPU (mean/std with eps by default, as
+    normalizer.cu:86-100), against both strands of the reference uploaded to
+    ctx.  Returns a POD5_RESULT_DTYPE array in file order; `target` is
+    min(fwd, rev) < threshold when a threshold is given."""
+    with Pod5File(path) as f:
+        info = f.reads()
+        sel = np.nonzero(info["n_samples"] >= skip + take)[0]
+        out = np.zeros(sel.size, POD5_RESULT_DTYPE)
+        if not sel.size:
+            return out
+        win, _ = f.signal(reads=sel, skip=skip, take=take, threads=threads)
+    normed, _ = ctx.normalize(list(win.astype(np.float32)), mode=norm, eps=eps)
+    sc, am = ctx.score(normed, **score_kw)
+    out["read_id"] = [info["read_id"][i].decode() for i in sel]
+    out["score"], out["argmin"] = sc, am
+    if threshold is not None:
+        out["target"] = sc.min(axis=1) < threshold
+    return out
+
+
 def read_fasta(path):
     """Sequence of a single-entry FASTA, as load_reference::ref_loader reads
     it (load_reference.hpp:164-212): lines after a '>' header are joined; an

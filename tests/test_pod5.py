@@ -347,3 +347,35 @@ def test_cli_mixed_fast5_and_pod5_folder(tmp_path):
     assert np.array_equal(rows["read_0083a4a2-b062-44ed-988e-e7f5a301121f"], sig[1952:4000].astype(np.float64))
     rid = "read_" + make_read(sig, 1)["read_id"]
     assert np.array_equal(rows[rid], sig[::-1][1952:4000].astype(np.float64))
+
+
+@pytest.mark.gpu
+def test_classify_pod5_matches_cli(tmp_path):
+    """classify_pod5 (Python) and dtwax_main --native (C++) score a POD5 the
+    same: same reads kept, same binary16 scores for both strands."""
+    import re
+    sig = np.load(os.path.join(GOLDEN, "batch0_signal.npy")).astype(np.int16)
+    rng = np.random.default_rng(4)
+    other = np.clip(rng.normal(480, 40, 5200), -32768, 32767).astype(np.int16)
+    d = tmp_path / "p"
+    d.mkdir()
+    reads = [make_read(sig, 0, read_id="0083a4a2-b062-44ed-988e-e7f5a301121f"), make_read(sig[:3000], 1),
+             make_read(other, 2)]
+    write_pod5(d / "r.pod5", reads)
+    model, fasta = os.path.join(GOLDEN, "dna_kmer_model.txt"), os.path.join(GOLDEN, "reference.fasta")
+    p = subprocess.run([CLI, str(d) + "/", model, fasta, "--native"], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    rows = [l.split("\t") for l in p.stdout.splitlines() if re.match(r"^\d+\tread_", l)]
+    ctx = D.Context(0)
+    try:
+        means = D.read_kmer_model(model)
+        levels, _ = D.squiggle_build(D.read_fasta(fasta), means, k=6)
+        ctx.upload_reference(levels)
+        res = D.classify_pod5(ctx, d / "r.pod5", threshold=150.0)
+    finally:
+        ctx.close()
+    assert [r[1] for r in rows] == ["read_" + x for x in res["read_id"]]
+    assert list(res["read_id"]) == [reads[0]["read_id"], reads[2]["read_id"]]  # the 3000-sample read is short
+    for r, got in zip(rows, res):  # the table prints 6 significant digits, as the reference's log
+        assert r[4] == f"{float(got['score'][0]):g}" and r[5] == f"{float(got['score'][1]):g}"
+    assert list(res["target"]) == list(res["score"].min(axis=1) < 150.0)
