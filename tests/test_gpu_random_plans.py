@@ -54,3 +54,18 @@ def test_random_plan_bit_exact(ctx, seed):
     osc = O.f16_to_f32(sc.ravel()).reshape(sc.shape)
     assert np.array_equal(res["score"], osc), (kw, lens)
     assert np.array_equal(res["argmin"], am), (kw, lens)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_plan_f32_bit_exact(ctx, seed):
+    """The binary32 form (the reference's FP32 build arithmetic) over random
+    widths and chunkings."""
+    rng, n_cols, lens, kw = _case(100 + seed)
+    kw = dict(segment=kw["segment"], ref_chunk=kw["ref_chunk"], precision="f32")
+    ref = rng.standard_normal((2, n_cols)).astype(np.float32)
+    ctx.upload_reference(ref)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in lens]
+    sc, am = ctx.score(reads, **kw)
+    osc, oam = O.batch_f32(reads, ref)
+    assert np.array_equal(sc, osc), (kw, lens)
+    assert np.array_equal(am, oam), (kw, lens)
