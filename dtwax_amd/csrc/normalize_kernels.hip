@@ -9,7 +9,9 @@
 //            order statistics (mean of the two middle ones for even n),
 //            found by radix selection on order-preserving integer keys
 //            (32-bit keys of the float samples, 64-bit keys of the FP64
-//            absolute deviations), 8 bits per pass with an LDS histogram.
+//            absolute deviations), 8 bits per pass with an LDS histogram
+//            and a parallel scan; the upper middle of an even count is
+//            found by one more pass (the next key), not a second selection.
 //   NONE:    copy.
 // Results are written as float; stats (optional) receive (centre, scale).
 
@@ -54,42 +56,106 @@ __device__ __forceinline__ double dkey_inv(uint64_t k) {
   return __builtin_bit_cast(double, u);
 }
 
-// k-th smallest (0-based) key over n elements, keys computed by `keyf(i)`.
+// Inclusive prefix sum over the workgroup of one value per thread (NT = 256).
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  uint32_t add = 0;
+  for (int i = 0; i < w; i++) add += wsum[i];
+  return v + add;
+}
+
+// Shared scratch of the order-statistic selection.
+struct SelectShared {
+  uint32_t hist[256];
+  uint32_t wsum[NT / 64];
+  uint64_t red[NT / 64];
+  uint64_t v[3];
+};
+
+// k-th smallest (0-based) key over n elements, keys computed by `keyf(i)`:
+// 8 bits per pass, an LDS histogram of the candidates' next digit, and a
+// parallel scan that finds the digit holding rank k.  *rank_eq receives k's
+// rank among the keys equal to the result and *n_eq their number.
 template <int BITS, class KeyF>
-__device__ uint64_t radix_select(int32_t n, int64_t k, KeyF keyf, uint32_t* hist, uint64_t* shv) {
+__device__ uint64_t radix_select(int32_t n, int64_t k, KeyF keyf, SelectShared& S, int64_t* rank_eq,
+                                 int64_t* n_eq) {
+  static_assert(NT == 256, "one histogram bin per thread");
   uint64_t prefix = 0, mask = 0;
+  int64_t eq = 0;
   for (int shift = BITS - 8; shift >= 0; shift -= 8) {
-    for (int i = threadIdx.x; i < 256; i += NT) hist[i] = 0;
+    S.hist[threadIdx.x] = 0;
     __syncthreads();
     for (int32_t i = threadIdx.x; i < n; i += NT) {
-      uint64_t key = keyf(i);
-      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 0xff], 1u);
+      const uint64_t key = keyf(i);
+      if ((key & mask) == prefix) atomicAdd(&S.hist[(key >> shift) & 0xff], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t acc = 0;
-      int bin = 0;
-      for (; bin < 256; bin++) {
-        if (acc + (int64_t)hist[bin] > k) break;
-        acc += hist[bin];
-      }
-      shv[0] = prefix | ((uint64_t)bin << shift);
-      shv[1] = (uint64_t)(k - acc);
+    const uint32_t c = S.hist[threadIdx.x];
+    const uint32_t incl = block_incl_scan(c, S.wsum);
+    const uint32_t excl = incl - c;
+    if ((int64_t)excl <= k && k < (int64_t)incl) {  // exactly one bin holds rank k
+      S.v[0] = prefix | ((uint64_t)threadIdx.x << shift);
+      S.v[1] = (uint64_t)(k - (int64_t)excl);
+      S.v[2] = c;
     }
     __syncthreads();
-    prefix = shv[0];
-    k = (int64_t)shv[1];
+    prefix = S.v[0];
+    k = (int64_t)S.v[1];
+    eq = (int64_t)S.v[2];
     mask |= (uint64_t)0xff << shift;
     __syncthreads();
   }
+  *rank_eq = k;
+  *n_eq = eq;
   return prefix;
+}
+
+// Smallest key above `key` (one pass and a workgroup min).
+template <class KeyF>
+__device__ uint64_t next_key(int32_t n, uint64_t key, KeyF keyf, SelectShared& S) {
+  uint64_t m = ~0ull;
+  for (int32_t i = threadIdx.x; i < n; i += NT) {
+    const uint64_t kk = keyf(i);
+    if (kk > key && kk < m) m = kk;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint64_t t = __shfl_xor(m, o, 64);
+    m = t < m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0) S.red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  uint64_t r = S.red[0];
+  for (int i = 1; i < NT / 64; i++) r = S.red[i] < r ? S.red[i] : r;
+  __syncthreads();
+  return r;
+}
+
+// The middle order statistics of n keys: (k, k) for odd n, (k - 1, k) for
+// even n, k = n / 2; the upper one of an even pair is the lower one again
+// when that key repeats, else the next key above it.
+template <int BITS, class KeyF>
+__device__ void middle_keys(int32_t n, KeyF keyf, SelectShared& S, uint64_t& lo, uint64_t& hi) {
+  int64_t rk, ne;
+  if (n & 1) {
+    lo = hi = radix_select<BITS>(n, n / 2, keyf, S, &rk, &ne);
+    return;
+  }
+  lo = radix_select<BITS>(n, n / 2 - 1, keyf, S, &rk, &ne);
+  hi = (rk + 1 < ne) ? lo : next_key(n, lo, keyf, S);
 }
 
 // Statistics of one read: centre, scale and the multiplier inv applied as
 // y = (float)((x - centre) * inv).  NONE gives centre 0, inv 1 (a copy).
 __device__ void read_stats(const float* __restrict__ x, int32_t n, int mode, double eps, double* sh,
-                           uint32_t* hist, uint64_t* shv, double& centre, double& scale,
-                           double& inv) {
+                           SelectShared& S, double& centre, double& scale, double& inv) {
   centre = 0.0;
   scale = 1.0;
   inv = 1.0;
@@ -108,23 +174,12 @@ __device__ void read_stats(const float* __restrict__ x, int32_t n, int mode, dou
     inv = 1.0 / scale;
   } else if (mode == DTWAX_NORM_MEDMAD) {
     auto kf = [&](int32_t i) -> uint64_t { return fkey(x[i]); };
-    double med;
-    if (n & 1) {
-      med = (double)fkey_inv((uint32_t)radix_select<32>(n, n / 2, kf, hist, shv));
-    } else {
-      double a = (double)fkey_inv((uint32_t)radix_select<32>(n, n / 2 - 1, kf, hist, shv));
-      double b = (double)fkey_inv((uint32_t)radix_select<32>(n, n / 2, kf, hist, shv));
-      med = 0.5 * (a + b);
-    }
+    uint64_t lo, hi;
+    middle_keys<32>(n, kf, S, lo, hi);
+    const double med = 0.5 * ((double)fkey_inv((uint32_t)lo) + (double)fkey_inv((uint32_t)hi));
     auto kd = [&](int32_t i) -> uint64_t { return dkey(fabs((double)x[i] - med)); };
-    double mad;
-    if (n & 1) {
-      mad = dkey_inv(radix_select<64>(n, n / 2, kd, hist, shv));
-    } else {
-      double a = dkey_inv(radix_select<64>(n, n / 2 - 1, kd, hist, shv));
-      double b = dkey_inv(radix_select<64>(n, n / 2, kd, hist, shv));
-      mad = 0.5 * (a + b);
-    }
+    middle_keys<64>(n, kd, S, lo, hi);
+    const double mad = 0.5 * (dkey_inv(lo) + dkey_inv(hi));
     centre = med;
     scale = 1.4826 * mad;
     inv = scale > 0.0 ? 1.0 / scale : 0.0;
@@ -138,15 +193,14 @@ __global__ __launch_bounds__(NT) void normalize_kernel(const float* __restrict__
                                                        float* __restrict__ out,
                                                        double* __restrict__ stats) {
   __shared__ double sh[NT / 64 + 1];
-  __shared__ uint32_t hist[256];
-  __shared__ uint64_t shv[2];
+  __shared__ SelectShared S;
   const int32_t r = blockIdx.x;
   const int32_t n = len[r];
   const float* x = raw + off[r];
   float* y = out + off[r];
   if (n <= 0) return;
   double centre, scale, inv;
-  read_stats(x, n, mode, eps, sh, hist, shv, centre, scale, inv);
+  read_stats(x, n, mode, eps, sh, S, centre, scale, inv);
   if (mode == DTWAX_NORM_NONE) {
     for (int32_t i = threadIdx.x; i < n; i += NT) y[i] = x[i];
   } else {
@@ -171,8 +225,7 @@ __global__ __launch_bounds__(NT) void ru_normalize_kernel(const float* __restric
                                                           double* __restrict__ slot_stats,
                                                           float* __restrict__ out) {
   __shared__ double sh[NT / 64 + 1];
-  __shared__ uint32_t hist[256];
-  __shared__ uint64_t shv[2];
+  __shared__ SelectShared S;
   __shared__ double cs[2];
   const int32_t c = blockIdx.x;
   const float* x = raw + (int64_t)c * chunk_len;
@@ -180,7 +233,7 @@ __global__ __launch_bounds__(NT) void ru_normalize_kernel(const float* __restric
   double* st = slot_stats + 2 * (int64_t)slot[c];
   if (first[c]) {
     double centre, scale, inv;
-    read_stats(x, chunk_len, mode, eps, sh, hist, shv, centre, scale, inv);
+    read_stats(x, chunk_len, mode, eps, sh, S, centre, scale, inv);
     if (threadIdx.x == 0) {
       st[0] = centre;
       st[1] = inv;

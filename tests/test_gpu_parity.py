@@ -128,6 +128,22 @@ def test_normalize_medmad_exact(ctx):
         assert np.array_equal(y, want)
 
 
+def test_normalize_medmad_ties(ctx):
+    """Integer ADC-like samples (many repeated values), even and odd counts,
+    constant reads (MAD 0) and two-valued reads: the middle order statistics
+    and the MAD stay exact when the middle keys repeat or differ."""
+    rng = np.random.default_rng(67)
+    reads = [rng.integers(400, 520, n).astype(np.float32) for n in (2, 4, 1000, 2000, 2001, 4096)]
+    reads += [np.full(10, 7.0, np.float32), np.full(11, -3.0, np.float32),
+              np.array([1, 1, 2, 2], np.float32), np.array([5, 5, 5, 9, 9, 9], np.float32),
+              np.repeat(np.float32([3, 8]), [999, 1001]), np.float32([0.0, -0.0, 0.0, -0.0])]
+    out, st = ctx.normalize(reads, mode=D.NORM_MEDMAD)
+    for r, y, (c, s) in zip(reads, out, st):
+        want, med, mad = O.norm_medmad(r)
+        assert c == med and s == 1.4826 * mad, (r[:8], c, med, s, mad)
+        assert np.array_equal(y, want)
+
+
 def test_normalize_meanstd(ctx):
     rng = np.random.default_rng(65)
     reads = [(rng.standard_normal(n) * 13 + 500).astype(np.float32) for n in (2, 1000, 2048)]
