@@ -23,7 +23,9 @@
 // Other columns (dictionary-encoded strings included) are stepped over by
 // their node and buffer counts.  A VBZ blob is a zstd frame of a 1-bit-key
 // stream-VByte block of zig-zag 16-bit differences, the row's `samples`
-// giving the count.  Every offset read from the file is bounds-checked.
+// giving the count.  A window of samples decodes only the rows it touches,
+// and in the row where it ends only the block's keys and leading data bytes
+// (zstd streaming).  Every offset read from the file is bounds-checked.
 
 #include <fcntl.h>
 #include <sys/mman.h>
