@@ -60,7 +60,7 @@ __device__ __forceinline__ h2 shfl_h2(h2 v, int src) {
 __global__ __launch_bounds__(64) void sdtw_compat_quirks_kernel(const uint32_t* __restrict__ ref_h2,
                                                                 const float* __restrict__ queries,
                                                                 int32_t n_tiles, DeviceResult* __restrict__ out) {
-  constexpr int S = 32, TILE = 832, PRE = 1024;
+  constexpr int S = 32, TILE = 832;
   const int lane = threadIdx.x;
   const float* q = queries + (int64_t)blockIdx.x * 2048;
   const h2 INF = (h2){(half_t)__builtin_inff(), (half_t)__builtin_inff()};
