@@ -77,6 +77,7 @@ def lib():
             "dtwax_plan_packing": ([vp, C.POINTER(i32), C.POINTER(i32)], C.c_int),
             "dtwax_plan_kernel_ms": ([vp, C.POINTER(f32)], C.c_int),
             "dtwax_distances": ([vp, vp, vp, vp, i32, C.c_uint32, vp, vp, i32], C.c_int),
+            "dtwax_distances_f32": ([vp, vp, vp, vp, i32, f32, vp, vp, i32], C.c_int),
             "dtwax_selftest_lanes": ([vp], C.c_int),
             "dtwax_autotune": ([vp, vp, vp, vp, i32, vp, i32, i64, C.c_char_p, vp, vp, i32,
                                 C.POINTER(i32)], C.c_int),
@@ -116,7 +117,7 @@ def exported_symbols():
         "dtwax_event_destroy", "dtwax_event_record", "dtwax_event_elapsed_ms",
         "dtwax_reference_upload", "dtwax_squiggle_build", "dtwax_normalize", "dtwax_score",
         "dtwax_plan_create", "dtwax_plan_destroy", "dtwax_plan_info", "dtwax_score_plan",
-        "dtwax_plan_set_timing", "dtwax_plan_packing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_selftest_lanes",
+        "dtwax_plan_set_timing", "dtwax_plan_packing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_distances_f32", "dtwax_selftest_lanes",
         "dtwax_autotune", "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
         "dtwax_ru_release", "dtwax_ru_start", "dtwax_ru_submit", "dtwax_ru_poll", "dtwax_ru_stop",
         "dtwax_vbz_decode", "dtwax_pod5_open", "dtwax_pod5_close", "dtwax_pod5_count",
@@ -305,10 +306,27 @@ class Context:
                                     C.byref(best), table.ctypes.data, len(table), C.byref(n)))
         return dict(segment=best.segment, ref_chunk=best.ref_chunk), table[: n.value]
 
-    def distances(self, d_ref_tiled, d_query_h2, d_dists, num_entries, compat_quirks=0,
-                  stream=None):
+    def distances(self, d_ref_tiled, d_query_h2, d_dists, num_entries, compat_quirks=1,
+                  stream=None, d_last_row=None):
+        """FullDTW::distances of the reference's FP16 build (DTW.hpp:285-295)
+        on device buffers in its layouts: tiled half2 reference, half2 query
+        words, half2 dists, optional last-row buffer (num_entries x 48128
+        half2).  compat_quirks: 1 = the reference kernel's DP (seam
+        deviations included), 0 = the clean recurrence, 2 = as 1 from the
+        tile-by-tile kernel.  Asynchronous on `stream`."""
         _check(lib().dtwax_distances(self.handle, d_ref_tiled.ptr, d_query_h2.ptr, d_dists.ptr,
-                                     num_entries, 0, stream, None, compat_quirks))
+                                     num_entries, 0, stream,
+                                     None if d_last_row is None else d_last_row.ptr, compat_quirks))
+
+    def distances_f32(self, d_ref_tiled, d_query, d_dists, num_entries, compat_quirks=1,
+                      stream=None, d_last_row=None):
+        """The same for the reference's FP32 build: tiled float reference of
+        the concatenated fwd | rev squiggle (96256 columns), float queries and
+        dists, optional last-row buffer (num_entries x 96256 floats)."""
+        _check(lib().dtwax_distances_f32(self.handle, d_ref_tiled.ptr, d_query.ptr, d_dists.ptr,
+                                         num_entries, 0.0, stream,
+                                         None if d_last_row is None else d_last_row.ptr,
+                                         compat_quirks))
 
 
 def squiggle_build(seq: str, model_means, ref_len=None, k=6):

@@ -76,8 +76,7 @@ void dtwax_destroy(dtwax_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   free_reference(c);
-  if (c->compat_plan) dtwax_plan_destroy(c->compat_plan);
-  for (void* p : {(void*)c->compat_q, (void*)c->compat_off, (void*)c->compat_len, (void*)c->compat_res})
+  for (void* p : {c->compat_ref[0], c->compat_ref[1], (void*)c->compat_counter, (void*)c->compat_keys})
     if (p) (void)hipFree(p);
   delete c;
 }
@@ -766,10 +765,20 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
   const int n_groups = L.n_groups ? L.n_groups : 1;
   HIP_TRY(launch_reset(L.counter, n_groups, L.keys, (int64_t)L.n_reads * 2, st));
   KParams k{};
-  k.ref = (L.precision == DTWAX_F32)
-              ? (const void*)((const float*)c->ref32_alloc + (int64_t)COL_PADL * c->n_strands)
-              : (const void*)((const uint32_t*)c->ref16_alloc + COL_PADL);
-  k.n_cols = (int32_t)c->n_cols;
+  if (L.ref) {
+    k.ref = L.ref;
+    k.n_cols = (int32_t)L.n_cols;
+  } else {
+    k.ref = (L.precision == DTWAX_F32)
+                ? (const void*)((const float*)c->ref32_alloc + (int64_t)COL_PADL * c->n_strands)
+                : (const void*)((const uint32_t*)c->ref16_alloc + COL_PADL);
+    k.n_cols = (int32_t)c->n_cols;
+  }
+  k.q_stride = L.q_stride;
+  k.last_row = L.last_row;
+  k.lr_stride = L.lr_stride;
+  k.lr_ntiles = L.lr_ntiles;
+  k.lr_last = L.lr_last;
   k.n_items = L.n_items;
   k.queries = L.queries;
   k.q_off = L.q_off;
@@ -793,7 +802,7 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
     hipStream_t gs = (g == 0 || !G.stream) ? st : G.stream;
     if (gs != st) HIP_TRY(hipStreamWaitEvent(gs, L.fork, 0));
     if (G.n_items > 0) {
-      k.items = L.items + G.item0;
+      k.items = L.items ? L.items + G.item0 : nullptr;
       k.n_items = G.n_items;
       k.counter = L.counter + g;
       k.renorm_k = G.renorm_k;
@@ -801,8 +810,9 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
       k.scratch_rows = G.rows;
       k.pack_lanes = L.pack_lanes;
       k.pack_reads = L.pack_reads;
-      HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, std::max(1, std::min(G.grid, G.n_items)), L.m3, gs,
-                          G.pack));
+      const int grid = std::max(1, std::min(G.grid, G.n_items));
+      if (L.fixed) HIP_TRY(launch_sdtw_fixed(k, L.precision, grid, L.quirks, gs));
+      else HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, grid, L.m3, gs, G.pack));
     }
     if (gs != st) {
       HIP_TRY(hipEventRecord(G.done, gs));
@@ -810,7 +820,8 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
     }
   }
   if (L.ev1) HIP_TRY(hipEventRecord(L.ev1, st));
-  HIP_TRY(launch_finalize(L.keys, L.n_reads, L.precision == DTWAX_F32 ? L.nstr : 2, L.out, st));
+  if (L.dists) HIP_TRY(launch_finalize_dists(L.keys, L.n_reads, L.dists_h2, L.dists, st));
+  else HIP_TRY(launch_finalize(L.keys, L.n_reads, L.precision == DTWAX_F32 ? L.nstr : 2, L.out, st));
   return DTWAX_OK;
 }
 }  // namespace dtwax
@@ -879,75 +890,101 @@ int dtwax_score(dtwax_ctx* c, const float* d_q, const int64_t* d_off, const int3
   return rc;
 }
 
-// ---- reference-compatible entry point ------------------------------------
-// Geometry of the reference build (src/include/common.hpp:60-98, -DFP16).
-static const int32_t kRefQuery = 2048, kRefSeg = 26, kRefWarp = 32, kRefLen = 48128;
+// ---- reference-layout entry points ----------------------------------------
+// Geometry of the reference builds (src/include/common.hpp:60-98): QUERY_LEN
+// 2048 rows; REF_LEN 48128 half2 {fwd, rev} columns (-DFP16) or 96256 float
+// columns of the concatenated fwd | rev squiggle (FP32 build); tiles of 832
+// columns, of which REF_LEN / 832 are scored (57 and 115); the last tile's
+// last-row entries sit at REF_LEN - 832 (DTW.cu:451, 546).
+namespace {
+constexpr int32_t kRefQuery = 2048;
+struct RefBuild {
+  bool fp16;
+  int32_t ref_len, n_tiles, n_cols;
+};
+constexpr RefBuild kBuildF16{true, 48128, 48128 / TILE_W, (48128 / TILE_W) * TILE_W};
+constexpr RefBuild kBuildF32{false, 96256, 96256 / TILE_W, (96256 / TILE_W) * TILE_W};
 
-int dtwax_distances(dtwax_ctx* c, const void* ref, const void* query, void* dists, int32_t num_entries,
-                    uint32_t thresh, void* stream, void* device_last_row, int32_t compat_quirks) {
-  (void)thresh;           // unused by the reference kernel as well (DTW.cu:152-153)
-  (void)device_last_row;  // band scratch is internal
+int distances_impl(dtwax_ctx* c, const RefBuild& g, const void* ref, const void* query, void* dists,
+                   int32_t num_entries, void* stream, void* device_last_row, int32_t compat_quirks) {
   if (!c || !ref || !query || !dists) return fail(DTWAX_EINVAL, "NULL argument");
   if (num_entries < 0) return fail(DTWAX_EINVAL, "num_entries < 0");
-  if (compat_quirks != 0 && compat_quirks != 1) return fail(DTWAX_EINVAL, "compat_quirks must be 0 or 1");
+  if (compat_quirks < 0 || compat_quirks > 2) return fail(DTWAX_EINVAL, "compat_quirks must be 0, 1 or 2");
   if (num_entries == 0) return DTWAX_OK;
   HIP_TRY(hipSetDevice(c->device));
   hipStream_t st = S_(stream);
-  const int32_t tile = kRefSeg * kRefWarp;
-  const int32_t n_tiles = kRefLen / tile;  // 57: columns beyond 57*832 are never scored
-  const int32_t n_cols = n_tiles * tile;
-  // reference: untile into the context's padded buffer (it becomes the current
-  // reference; the stored one is replaced, as the caller passes it every call)
-  if (c->n_cols != n_cols || c->n_strands != 2 || !c->ref16_alloc) {
-    std::vector<float> zeros((size_t)2 * n_cols, 0.0f);
-    int rc = dtwax_reference_upload(c, zeros.data(), 2, n_cols);
-    if (rc) return rc;
-  }
-  HIP_TRY(launch_untile_reference((const uint32_t*)ref, kRefLen, kRefSeg, kRefWarp, n_tiles,
-                                  (uint32_t*)c->ref16_alloc + COL_PADL, st));
-  // queries
-  const int64_t nq = (int64_t)num_entries * kRefQuery;
-  if (c->compat_cap < num_entries) {
-    for (void* p : {(void*)c->compat_q, (void*)c->compat_off, (void*)c->compat_len, (void*)c->compat_res})
-      if (p) (void)hipFree(p);
-    if (hipMalloc(&c->compat_q, (size_t)nq * 4) != hipSuccess ||
-        hipMalloc(&c->compat_off, (size_t)num_entries * 8) != hipSuccess ||
-        hipMalloc(&c->compat_len, (size_t)num_entries * 4) != hipSuccess ||
-        hipMalloc(&c->compat_res, (size_t)num_entries * sizeof(DeviceResult)) != hipSuccess) {
-      c->compat_cap = 0;
-      return fail(DTWAX_ENOMEM, "compat buffers");
+  // The untiled reference in the context's own buffer for this build (the
+  // reference uploaded with dtwax_reference_upload is left alone), padded
+  // like it so the sweep reads past both ends without bounds checks.
+  void*& rbuf = c->compat_ref[g.fp16 ? 0 : 1];
+  if (!rbuf) {
+    const int64_t total = COL_PADL + g.n_cols + COL_PADR;
+    if (hipMalloc(&rbuf, (size_t)total * 4) != hipSuccess) {
+      rbuf = nullptr;
+      return fail(DTWAX_ENOMEM, "reference-layout reference buffer");
     }
-    std::vector<int64_t> off((size_t)num_entries);
-    std::vector<int32_t> len((size_t)num_entries, kRefQuery);
-    for (int32_t i = 0; i < num_entries; i++) off[(size_t)i] = (int64_t)i * kRefQuery;
-    HIP_TRY(hipMemcpy(c->compat_off, off.data(), off.size() * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->compat_len, len.data(), len.size() * 4, hipMemcpyHostToDevice));
-    c->compat_cap = num_entries;
+    const uint16_t ph = __builtin_bit_cast(uint16_t, kPadF16);
+    const uint32_t pad = g.fp16 ? ((uint32_t)ph | ((uint32_t)ph << 16)) : __builtin_bit_cast(uint32_t, kPadF32);
+    HIP_TRY(launch_fill_u32((uint32_t*)rbuf, total, pad, st));
   }
-  HIP_TRY(launch_half2_queries_to_float((const uint32_t*)query, nq, c->compat_q, st));
-  if (compat_quirks) {
-    HIP_TRY(launch_compat_quirks((const uint32_t*)c->ref16_alloc + COL_PADL, c->compat_q, num_entries, n_tiles,
-                                 c->compat_res, st));
-    HIP_TRY(launch_results_to_half2(c->compat_res, num_entries, (uint32_t*)dists, st));
+  uint32_t* flat = (uint32_t*)rbuf + COL_PADL;
+  HIP_TRY(launch_untile_reference((const uint32_t*)ref, g.n_tiles, flat, st));
+  const int32_t lr_last = g.ref_len - TILE_W;
+  if (compat_quirks == 2 || (compat_quirks == 1 && !g.fp16)) {
+    // the reference's tile-by-tile schedule (the only form with the
+    // deviations for the FP32 build)
+    HIP_TRY(launch_compat_tiles(g.fp16, flat, query, num_entries, g.n_tiles, dists, device_last_row, g.ref_len,
+                                lr_last, st));
     return DTWAX_OK;
   }
-  if (!c->compat_plan || c->compat_plan_n != num_entries) {
-    if (c->compat_plan) dtwax_plan_destroy(c->compat_plan);
-    c->compat_plan = nullptr;
-    std::vector<int32_t> lens((size_t)num_entries, kRefQuery);
-    dtwax_score_opts o{};
-    o.precision = DTWAX_F16;
-    o.segment = 32;  // 64 lanes x 32 rows = QUERY_LEN: one band, no padding rows
-    o.ref_chunk = n_cols;
-    int rc = dtwax_plan_create(c, lens.data(), num_entries, &o, &c->compat_plan);
-    if (rc) return rc;
-    c->compat_plan_n = num_entries;
+  if (c->compat_cap < num_entries) {
+    if (c->compat_keys) (void)hipFree(c->compat_keys);
+    if (c->compat_counter) (void)hipFree(c->compat_counter);
+    c->compat_keys = nullptr;
+    c->compat_counter = nullptr;
+    c->compat_cap = 0;
+    if (hipMalloc(&c->compat_keys, (size_t)num_entries * 2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->compat_counter, 256) != hipSuccess)
+      return fail(DTWAX_ENOMEM, "reference-layout keys");
+    c->compat_cap = num_entries;
   }
-  int rc = run_plan(c->compat_plan, c->compat_q, c->compat_off, c->compat_len,
-                    reinterpret_cast<dtwax_result*>(c->compat_res), st);
-  if (rc) return rc;
-  HIP_TRY(launch_results_to_half2(c->compat_res, num_entries, (uint32_t*)dists, st));
-  return DTWAX_OK;
+  const bool quirks = compat_quirks == 1;
+  const int occ = occupancy_fixed(g.fp16 ? DTWAX_F16 : DTWAX_F32, quirks);
+  ScoreLaunch L;
+  L.fixed = true;
+  L.precision = g.fp16 ? DTWAX_F16 : DTWAX_F32;
+  L.S = 32;  // 64 lanes x 32 rows = QUERY_LEN: one band, no band scratch
+  L.nstr = g.fp16 ? 2 : 1;
+  L.grid = std::max(1, std::min(num_entries, std::max(1, occ) * c->n_cu));
+  L.queries = (const float*)query;
+  L.q_stride = kRefQuery;
+  L.n_items = num_entries;
+  L.n_reads = num_entries;
+  L.counter = c->compat_counter;
+  L.keys = c->compat_keys;
+  L.ref = flat;
+  L.n_cols = g.n_cols;
+  L.quirks = quirks;
+  L.last_row = device_last_row;
+  L.lr_stride = g.ref_len;
+  L.lr_ntiles = g.n_tiles;
+  L.lr_last = lr_last;
+  L.dists = dists;
+  L.dists_h2 = g.fp16;
+  return run_score_launch(c, L, st);
+}
+}  // namespace
+
+int dtwax_distances(dtwax_ctx* c, const void* ref, const void* query, void* dists, int32_t num_entries,
+                    uint32_t thresh, void* stream, void* device_last_row, int32_t compat_quirks) {
+  (void)thresh;  // unused by the reference kernel as well (DTW.cu:152-153)
+  return distances_impl(c, kBuildF16, ref, query, dists, num_entries, stream, device_last_row, compat_quirks);
+}
+
+int dtwax_distances_f32(dtwax_ctx* c, const void* ref, const void* query, void* dists, int32_t num_entries,
+                        float thresh, void* stream, void* device_last_row, int32_t compat_quirks) {
+  (void)thresh;
+  return distances_impl(c, kBuildF32, ref, query, dists, num_entries, stream, device_last_row, compat_quirks);
 }
 
 // ---- self test ---------------------------------------------------------------

@@ -17,8 +17,13 @@
 //   * table: j, read group, QUERY_LEN, REF_LEN, fwd score, rev score.
 // Defaults reproduce the reference's compile-time geometry (QUERY_LEN 2048,
 // ADAPTER_LEN 1952, REF_LEN 48128 of which 57*832 = 47,424 columns are
-// scored).  Options after the positionals select the native mode (whole
-// reference, any query length, medmad, other precisions).
+// scored) and its launch (main.cu:92-121, 165-204): the squiggle tiled on the
+// host, queries rounded to half2 {q, q} with 32 zero pad words, scored by
+// dtwax_distances (dtwax_distances_f32 with --fp32-build), whose DP is the
+// reference kernel's own, seam deviations included; --clean scores the same
+// launch with the clean recurrence.  Options after the positionals select the
+// native mode (whole reference, any query length, medmad, other precisions),
+// scored by dtwax_score.
 // Exit status is non-zero on any error (the reference exits on the first
 // CUDA error and on a missing VBZ plugin).
 //
@@ -39,6 +44,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -67,14 +73,15 @@ struct Options {
   bool dump_reads = false;   // print loaded raw samples and stop (no GPU)
   int device = 0;
   bool fp32_build = false;   // the reference's FP32 build: concatenated fwd|rev, one score
+  bool clean = false;        // reference launch, clean recurrence (no seam deviations)
 };
 
 void usage() {
   std::cerr << "usage: dtwax_main <fast5_folder/> <kmer_model> <ref.fasta> [--native] [--query-len N]\n"
                "       [--adapter-len N] [--precision f16|f16_fused|f16_renorm|f32] [--norm meanstd|medmad]\n"
-               "       [--fp32-build] [--device N] [--dump-reads]\n"
+               "       [--fp32-build] [--clean] [--device N] [--dump-reads]\n"
                "   or: dtwax_main <kmer_model> <ref.fasta> [pipe] [--reads N] [--query-len N] [--native]\n"
-               "       [--precision ...] [--norm ...] [--device N] [--keep-open]   (Read Until FIFO loop)\n";
+               "       [--precision ...] [--norm ...] [--clean] [--device N] [--keep-open]   (Read Until FIFO loop)\n";
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -109,7 +116,8 @@ bool parse(int argc, char** argv, Options& o) {
     } else if (a == "--fp32-build") {
       o.fp32_build = true;
       o.precision = DTWAX_F32;
-    } else if (a == "--device") o.device = std::atoi(next("--device"));
+    } else if (a == "--clean") o.clean = true;
+    else if (a == "--device") o.device = std::atoi(next("--device"));
     else if (a == "--dump-reads") o.dump_reads = true;
     else if (a == "--help" || a == "-h") return false;
     else if (i == 4 && a == "pipe") continue;  // tolerated like the RU driver's argv[3]
@@ -251,6 +259,107 @@ bool load_model(const std::string& path, int& k, std::vector<float>& means) {
     }                                                                             \
   } while (0)
 
+// float -> binary16 bits, round to nearest even (__float2half_rn semantics).
+uint16_t f32_to_f16_rn(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return (uint16_t)(sign | (ax > 0x7f800000u ? 0x7e00u : 0x7c00u));
+  if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // >= 65520 rounds to inf
+  if (ax >= 0x38800000u) {                                     // normal half
+    const uint32_t r = ax + 0xfffu + ((ax >> 13) & 1u);
+    return (uint16_t)(sign | ((r - 0x38000000u) >> 13));
+  }
+  float a;  // subnormal half: units of 2^-24, exact scaling then RN to an integer
+  std::memcpy(&a, &ax, 4);
+  return (uint16_t)(sign | (uint32_t)std::nearbyint(a * 16777216.0f));
+}
+
+// The reference's launch on the mirror entry points (src/main.cu:78-121,
+// 165-204, 211-271): reference built once in the tiled layout
+// h[l*832 + i*32 + j] = flat[l*832 + j*26 + i] over REF_LEN / 832 tiles
+// (binary16 {fwd, rev} words, load_reference.hpp:93-97, or the concatenated
+// fwd | rev floats of the FP32 build, :35-37); per batch the normalised
+// queries as {q, q} half2 words (or floats) with WARP_SIZE zero pad words
+// (main.cu:165-179), one distances call, the dists copied back.
+struct RefMirror {
+  bool fp32 = false;
+  int64_t ref_len = 0;  // REF_LEN: 48128 half2 words or 96256 floats
+  void *d_ref = nullptr, *d_q = nullptr, *d_dists = nullptr;
+  int cap = 0;
+  void init(dtwax_ctx* ctx, const std::vector<float>& levels, int64_t per_strand, bool fp32_build) {
+    (void)ctx;
+    fp32 = fp32_build;
+    ref_len = fp32 ? 2 * per_strand : per_strand;
+    std::vector<uint32_t> flat((size_t)ref_len), tiled((size_t)ref_len, 0u);
+    for (int64_t i = 0; i < ref_len; i++) {
+      if (fp32) {
+        std::memcpy(&flat[(size_t)i], &levels[(size_t)i], 4);  // fwd then rev, as in levels
+      } else {
+        flat[(size_t)i] = (uint32_t)f32_to_f16_rn(levels[(size_t)i]) |
+                          ((uint32_t)f32_to_f16_rn(levels[(size_t)(per_strand + i)]) << 16);
+      }
+    }
+    const int64_t tile = 26 * 32, n_tiles = ref_len / tile;
+    for (int64_t l = 0; l < n_tiles; l++)
+      for (int i = 0; i < 26; i++)
+        for (int j = 0; j < 32; j++) tiled[(size_t)(l * tile + i * 32 + j)] = flat[(size_t)(l * tile + j * 26 + i)];
+    CHECK(dtwax_malloc(&d_ref, (size_t)ref_len * 4));
+    CHECK(dtwax_memcpy_h2d(d_ref, tiled.data(), (size_t)ref_len * 4, nullptr));
+  }
+  // q: n normalised queries of 2048 samples -> out: n x (fp32 ? 1 : 2) scores
+  void score(dtwax_ctx* ctx, const float* q, int n, bool clean, std::vector<float>& out) {
+    const size_t nq = (size_t)n * 2048 + 32;
+    if (n > cap) {
+      if (d_q) dtwax_free(d_q);
+      if (d_dists) dtwax_free(d_dists);
+      CHECK(dtwax_malloc(&d_q, nq * 4));
+      CHECK(dtwax_malloc(&d_dists, (size_t)n * 4));
+      cap = n;
+    }
+    std::vector<uint32_t> words(nq, 0u);
+    for (size_t i = 0; i + 32 < nq; i++) {
+      if (fp32) {
+        std::memcpy(&words[i], &q[i], 4);
+      } else {
+        const uint32_t h = f32_to_f16_rn(q[i]);
+        words[i] = h | (h << 16);
+      }
+    }
+    CHECK(dtwax_memcpy_h2d(d_q, words.data(), nq * 4, nullptr));
+    const int quirks = clean ? 0 : 1;
+    if (fp32) CHECK(dtwax_distances_f32(ctx, d_ref, d_q, d_dists, n, 0.0f, nullptr, nullptr, quirks));
+    else CHECK(dtwax_distances(ctx, d_ref, d_q, d_dists, n, 0u, nullptr, nullptr, quirks));
+    std::vector<uint32_t> d((size_t)n);
+    CHECK(dtwax_memcpy_d2h(d.data(), d_dists, (size_t)n * 4, nullptr));
+    out.assign((size_t)n * (fp32 ? 1 : 2), 0.0f);
+    for (int j = 0; j < n; j++) {
+      if (fp32) {
+        std::memcpy(&out[(size_t)j], &d[(size_t)j], 4);
+      } else {
+        out[(size_t)(2 * j)] = f16_to_f32((uint16_t)(d[(size_t)j] & 0xffffu));
+        out[(size_t)(2 * j + 1)] = f16_to_f32((uint16_t)(d[(size_t)j] >> 16));
+      }
+    }
+  }
+  void release() {
+    for (void* p : {d_ref, d_q, d_dists})
+      if (p) dtwax_free(p);
+    d_ref = d_q = d_dists = nullptr;
+  }
+  static float f16_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    float v;
+    if (e == 31) {
+      const uint32_t b = sign | 0x7f800000u | (m << 13);
+      std::memcpy(&v, &b, 4);
+      return v;
+    }
+    v = e ? std::ldexp((float)(m | 0x400u), (int)e - 25) : std::ldexp((float)m, -24);
+    return sign ? -v : v;
+  }
+};
+
 bool is_directory(const char* p) {
   struct stat st;
   return stat(p, &st) == 0 && S_ISDIR(st.st_mode);
@@ -260,7 +369,7 @@ bool is_directory(const char* p) {
 int ru_main(int argc, char** argv) {
   std::string model = argv[1], fasta = argv[2], pipe = "pipe";
   int reads = 10, query_len = 2048, device = 0, precision = DTWAX_F16, norm = DTWAX_NORM_MEANSTD;
-  bool native = false, keep_open = false;
+  bool native = false, keep_open = false, clean = false;
   for (int i = 3; i < argc; i++) {
     std::string a = argv[i];
     auto next = [&]() -> const char* {
@@ -276,6 +385,7 @@ int ru_main(int argc, char** argv) {
     else if (a == "--device") device = std::atoi(next());
     else if (a == "--native") native = true;
     else if (a == "--keep-open") keep_open = true;
+    else if (a == "--clean") clean = true;
     else if (a == "--precision") {
       std::string p = next();
       if (p == "f16") precision = DTWAX_F16;
@@ -315,6 +425,10 @@ int ru_main(int argc, char** argv) {
   const int64_t n_cols = native ? ref_len : std::min<int64_t>(47424, ref_len);
   dtwax_ctx* ctx = nullptr;
   CHECK(dtwax_create(device, &ctx));
+  // the reference's own launch at its geometry (ru_main.cu runs the same kernel)
+  const bool mirror = !native && query_len == 2048 && precision == DTWAX_F16;
+  RefMirror rm;
+  if (mirror) rm.init(ctx, levels, ref_len, false);
   {
     std::vector<float> two((size_t)(2 * n_cols));
     std::copy(levels.begin(), levels.begin() + n_cols, two.begin());
@@ -338,7 +452,7 @@ int ru_main(int argc, char** argv) {
   dtwax_plan* plan = nullptr;
   CHECK(dtwax_plan_create(ctx, lens.data(), reads, &opt, &plan));
   std::vector<unsigned char> bytes(batch);
-  std::vector<float> raw(batch);
+  std::vector<float> raw(batch), normed(batch), sc;
   std::vector<dtwax_result> res((size_t)reads);
   for (;;) {
     std::FILE* f = std::fopen(pipe.c_str(), "rb");  // blocks until a writer opens the FIFO
@@ -353,9 +467,18 @@ int ru_main(int argc, char** argv) {
       CHECK(dtwax_memcpy_h2d(d_raw, raw.data(), batch * 4, nullptr));
       CHECK(dtwax_normalize(ctx, (const float*)d_raw, (const int64_t*)d_off, (const int32_t*)d_len, reads, norm,
                             1e-4, (float*)d_norm, nullptr, nullptr));
-      CHECK(dtwax_score_plan(plan, (const float*)d_norm, (const int64_t*)d_off, (const int32_t*)d_len,
-                             (dtwax_result*)d_out, nullptr));
-      CHECK(dtwax_memcpy_d2h(res.data(), d_out, res.size() * sizeof(dtwax_result), nullptr));
+      if (mirror) {
+        CHECK(dtwax_memcpy_d2h(normed.data(), d_norm, batch * 4, nullptr));
+        rm.score(ctx, normed.data(), reads, clean, sc);
+        for (int j = 0; j < reads; j++) {
+          res[(size_t)j].score[0] = sc[(size_t)(2 * j)];
+          res[(size_t)j].score[1] = sc[(size_t)(2 * j + 1)];
+        }
+      } else {
+        CHECK(dtwax_score_plan(plan, (const float*)d_norm, (const int64_t*)d_off, (const int32_t*)d_len,
+                               (dtwax_result*)d_out, nullptr));
+        CHECK(dtwax_memcpy_d2h(res.data(), d_out, res.size() * sizeof(dtwax_result), nullptr));
+      }
       for (int j = 0; j < reads; j++)
         std::cout << j << "\t" << j << "\t" << query_len << "\t" << ref_len << "\t" << res[(size_t)j].score[0]
                   << "\t" << res[(size_t)j].score[1] << "\n";
@@ -365,6 +488,7 @@ int ru_main(int argc, char** argv) {
     if (!keep_open) break;
   }
   dtwax_plan_destroy(plan);
+  rm.release();
   for (void* p : {d_raw, d_norm, d_off, d_len, d_out}) dtwax_free(p);
   dtwax_destroy(ctx);
   return 0;
@@ -397,10 +521,16 @@ int main(int argc, char** argv) {
   int64_t skipped = 0;
   CHECK(dtwax_squiggle_build(seq.data(), (int64_t)seq.size(), k, means.data(), ref_len, levels.data(), &skipped));
   int64_t n_cols = o.native ? ref_len : std::min<int64_t>(o.scored_cols, ref_len);
+  // the reference's own launch: its geometry, the binary16 or FP32 build
+  const bool mirror = !o.native && o.query_len == 2048 && ref_len == 48128 &&
+                      (o.fp32_build || o.precision == DTWAX_F16);
+  RefMirror rm;
   dtwax_ctx* ctx = nullptr;
   if (!o.dump_reads) {
     CHECK(dtwax_create(o.device, &ctx));
-    if (o.fp32_build) {
+    if (mirror) {
+      rm.init(ctx, levels, ref_len, o.fp32_build);
+    } else if (o.fp32_build) {
       // FP32 build: one sequence fwd[0:REF_LEN) | rev[0:REF_LEN), tiles of 832
       // over the concatenation (load_reference.hpp:35-37, common.hpp:70-73)
       const int64_t total = 2 * ref_len;
@@ -505,14 +635,31 @@ int main(int argc, char** argv) {
     std::cout << "Issuing " << in_batch << " reads (blocks) from base addr:" << (int64_t)b * block_num * o.query_len
               << " to stream_id 0\n";
   }
+  std::vector<float> normed;
+  if (mirror && n) {  // the reference converts the normalised reads on the host (main.cu:165-179)
+    normed.resize(raw.size());
+    CHECK(dtwax_memcpy_d2h(normed.data(), d_norm, nbytes, nullptr));
+  }
   Timer t_dtw;
-  std::vector<dtwax_result> res((size_t)n);
-  if (n) {
+  std::vector<float> fwd((size_t)n), rev((size_t)n);
+  if (n && mirror) {
+    std::vector<float> sc;
+    rm.score(ctx, normed.data(), n, o.clean, sc);
+    for (int32_t j = 0; j < n; j++) {
+      fwd[(size_t)j] = o.fp32_build ? sc[(size_t)j] : sc[(size_t)(2 * j)];
+      rev[(size_t)j] = o.fp32_build ? 0.0f : sc[(size_t)(2 * j + 1)];
+    }
+  } else if (n) {
+    std::vector<dtwax_result> res((size_t)n);
     dtwax_score_opts opt{};
     opt.precision = o.precision;
     CHECK(dtwax_score(ctx, (const float*)d_norm, (const int64_t*)d_off, (const int32_t*)d_len, n, &opt,
                       (dtwax_result*)d_out, nullptr));
     CHECK(dtwax_memcpy_d2h(res.data(), d_out, res.size() * sizeof(dtwax_result), nullptr));
+    for (int32_t j = 0; j < n; j++) {
+      fwd[(size_t)j] = res[(size_t)j].score[0];
+      rev[(size_t)j] = res[(size_t)j].score[1];
+    }
   }
   const double dtw_ms = t_dtw.ms();
   std::cout << "TIMING: " << dtw_ms << " ms " << ((o.query_len / (dtw_ms * 1e3)) * n / 10)
@@ -524,14 +671,15 @@ int main(int argc, char** argv) {
     std::cout << "Read_ID\t" << "QUERY_LEN\t" << "REF_LEN\t" << "sDTW-score\n";
     for (int32_t j = 0; j < n; j++)
       std::cout << j << "\t" << ids[(size_t)j] << "\t" << o.query_len << "\t" << ref_len_printed << "\t"
-                << res[(size_t)j].score[0] << "\n";
+                << fwd[(size_t)j] << "\n";
   } else {
     std::cout << "Read_ID\t" << "QUERY_LEN\t" << "REF_LEN\t" << "sDTW score: fwd-strand\tsDTW score: rev-strand\n";
     for (int32_t j = 0; j < n; j++)
       std::cout << j << "\t" << ids[(size_t)j] << "\t" << o.query_len << "\t" << ref_len_printed << "\t"
-                << res[(size_t)j].score[0] << "\t" << res[(size_t)j].score[1] << "\n";
+                << fwd[(size_t)j] << "\t" << rev[(size_t)j] << "\n";
   }
   Timer t_free;
+  rm.release();
   for (void* p : {d_raw, d_norm, d_off, d_len, d_out}) dtwax_free(p);
   dtwax_destroy(ctx);
   timing("free", t_free.ms());

@@ -64,6 +64,20 @@ struct ScoreLaunch {
   const LaunchGroup* groups = nullptr;  // counter[g] for group g
   int n_groups = 0;
   hipEvent_t fork = nullptr;  // recorded on the caller's stream before the groups start
+  // Reference-layout batches (dtwax_distances*, launch_sdtw_fixed): a
+  // reference other than the context's (column 0 of a padded buffer, n_cols
+  // columns), fixed-stride queries, the seam deviations, the tiled last row,
+  // and results in the reference's dists layout instead of `out`.
+  bool fixed = false;
+  const void* ref = nullptr;
+  int64_t n_cols = 0;
+  int32_t q_stride = 0;
+  bool quirks = false;
+  void* last_row = nullptr;
+  int64_t lr_stride = 0;
+  int32_t lr_ntiles = 0, lr_last = 0;
+  void* dists = nullptr;
+  bool dists_h2 = false;
 };
 
 }  // namespace dtwax
@@ -78,14 +92,15 @@ struct dtwax_ctx {
   int64_t n_cols = 0;
   void* ref16_alloc = nullptr;  // h2 per column, COL_PADL + n + COL_PADR
   void* ref32_alloc = nullptr;  // n_strands floats per column, same padding
-  // compat entry point buffers
-  float* compat_q = nullptr;
-  int64_t* compat_off = nullptr;
-  int32_t* compat_len = nullptr;
-  dtwax::DeviceResult* compat_res = nullptr;
+  // Buffers of the reference-layout entry points (dtwax_distances for the
+  // FP16 build, dtwax_distances_f32 for the FP32 build), apart from the
+  // uploaded reference: the untiled reference of each (padded like it),
+  // allocated on first use, and one work counter + 2 keys per read, grown
+  // to the largest batch.
+  void* compat_ref[2] = {nullptr, nullptr};  // [0] half2, [1] float
+  int32_t* compat_counter = nullptr;
+  unsigned long long* compat_keys = nullptr;
   int32_t compat_cap = 0;
-  struct dtwax_plan* compat_plan = nullptr;
-  int32_t compat_plan_n = -1;
 };
 
 // A plan: work items (read x reference chunk) in decreasing cost order and

@@ -86,12 +86,36 @@ struct KParams {
   const PackLane* pack_lanes;  // 64 per pack (packed launches)
   const PackRead* pack_reads;
   int32_t scratch_rows;        // band scratch rows per workgroup (packed launches; else 1)
+  // Fixed-geometry kernels (launch_sdtw_fixed): read i is q_stride samples
+  // from i * q_stride (q_off, q_len and items unused).
+  int32_t q_stride;
+  // Fixed-geometry kernels: when not null, the final band's last row also goes to
+  // last_row[read * lr_stride + tiled(col)], the reference's tiled layout:
+  // tile t = col / TILE_W at t * TILE_W (the last of lr_ntiles tiles at
+  // lr_last), column 26 j + i of a tile at j + 32 i.
+  void* last_row;
+  int64_t lr_stride;
+  int32_t lr_ntiles;
+  int32_t lr_last;
 };
+
+// The reference's tile geometry (src/include/common.hpp:63-66, 87):
+// SEGMENT_SIZE 26 columns per lane, WARP_SIZE 32 lanes, tiles of 832 columns.
+constexpr int REF_SEG = 26;
+constexpr int REF_WARP = 32;
+constexpr int TILE_W = REF_SEG * REF_WARP;
 
 // m3: three moves (the left move D[r][c-1] too; DTWAX_F16 and DTWAX_F32 only)
 // pack: the packed kernel (DTWAX_F16 / DTWAX_F16_FUSED, S = 16, 20, 25, 28 or 32, two moves)
 hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool m3,
                        hipStream_t st, bool pack = false);
+// The fixed-geometry kernels of the reference-layout entry points: S = 32,
+// two moves, every read q_stride <= 2048 rows (one band), items implicit
+// (item i = read i over [0, n_cols) from x = 0); DTWAX_F16 reads half2 query
+// words and may apply the reference kernel's seam deviations (quirks);
+// DTWAX_F32 is one strand of float queries.
+hipError_t launch_sdtw_fixed(const KParams& p, int precision, int grid, bool quirks, hipStream_t st);
+int occupancy_fixed(int precision, bool quirks);
 // zero n_counters work counters (n_counters <= 256) and set n_keys keys to ~0
 hipError_t launch_reset(int32_t* counters, int32_t n_counters, unsigned long long* keys, int64_t n_keys,
                         hipStream_t st);
@@ -115,14 +139,21 @@ hipError_t launch_ru_normalize(const float* raw, int32_t n_chunks, int32_t chunk
 // fill n 32-bit words with `value` (a kernel, so it may be captured in a graph)
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, hipStream_t st);
 
+// keys -> the reference's output layout: one half2 word {fwd, rev} per read
+// (h2 = true, the FP16 build) or one float per read (the FP32 build, strand 0)
+hipError_t launch_finalize_dists(const unsigned long long* keys, int32_t n_reads, bool h2, void* dists,
+                                 hipStream_t st);
+
 // reference-compat helpers (compat_kernels.hip)
-hipError_t launch_untile_reference(const uint32_t* tiled, int32_t ref_len, int32_t seg, int32_t warp,
-                                   int32_t n_tiles, uint32_t* untiled_h2, hipStream_t st);
-hipError_t launch_half2_queries_to_float(const uint32_t* q_h2, int64_t n, float* out, hipStream_t st);
-hipError_t launch_results_to_half2(const DeviceResult* res, int32_t n, uint32_t* dists, hipStream_t st);
-// binary16 scores with the reference kernel's seam deviations (QUERY_LEN 2048,
-// PREFIX_LEN 1024, tiles of 832 columns): ref_h2 = flat {fwd, rev} columns
-hipError_t launch_compat_quirks(const uint32_t* ref_h2, const float* queries, int32_t n_reads, int32_t n_tiles,
-                                DeviceResult* out, hipStream_t st);
+// tiled 32-bit words (half2 or float; src/main.cu:92-109) -> flat column order
+hipError_t launch_untile_reference(const uint32_t* tiled, int32_t n_tiles, uint32_t* flat, hipStream_t st);
+// The reference kernel's schedule itself, one wave per read draining at every
+// tile end (QUERY_LEN 2048, PREFIX_LEN 1024, tiles of 832 columns), with its
+// seam deviations: ref = flat columns (half2 {fwd, rev} words when h2, else
+// floats), queries = num_entries x 2048 half2 words {q, q} (h2) or floats;
+// dists and last_row (optional) in the reference's layouts.
+hipError_t launch_compat_tiles(bool h2, const uint32_t* ref, const void* queries, int32_t n_reads,
+                               int32_t n_tiles, void* dists, void* last_row, int64_t lr_stride,
+                               int32_t lr_last, hipStream_t st);
 
 }  // namespace dtwax

@@ -101,6 +101,46 @@ __device__ __forceinline__ void publish_min(unsigned long long* addr, uint64_t k
   atomicMin(addr, lane == 0 ? (unsigned long long)key : ~0ull);
 }
 
+// FIX selects the fixed-geometry form of a kernel (the reference-layout
+// entry points): item i is read i over the whole strand [0, n_cols) from
+// x = 0, read i is q_stride samples from i * q_stride, binary16 queries are
+// half2 words {q_fwd, q_rev}, and the final band's last row also goes to
+// KParams::last_row in the reference's tiled layout.  The plan kernels
+// (FIX = false) read the plan's items, offsets and lengths and compile
+// without any of it.
+template <bool FIX>
+__device__ __forceinline__ WorkItem load_item(const KParams& p, int idx) {
+  if constexpr (!FIX) {
+    return p.items[idx];
+  } else {
+    WorkItem it{};
+    it.read = idx;
+    it.x = 0;
+    it.c0 = 0;
+    it.c1 = p.n_cols;
+    return it;
+  }
+}
+// First sample and length of a read.
+template <bool FIX>
+__device__ __forceinline__ int64_t read_offset(const KParams& p, int32_t read) {
+  if constexpr (FIX) return (int64_t)read * p.q_stride;
+  else return p.q_off[read];
+}
+template <bool FIX>
+__device__ __forceinline__ int32_t read_length(const KParams& p, int32_t read) {
+  if constexpr (FIX) return p.q_stride;
+  else return p.q_len[read];
+}
+// Index of column col in the reference's tiled row layout (a tile of 832
+// columns holds lane j's 26 columns j*26 + i at j + 32*i; the last tile sits
+// at lr_last, src/include/DTW.cu:293, :419, :546).
+__device__ __forceinline__ int64_t tiled_index(int32_t ntiles, int32_t last, int32_t col) {
+  const int32_t t = col / TILE_W, w = col - t * TILE_W;
+  const int32_t j = w / REF_SEG, i = w - j * REF_SEG;
+  return (int64_t)(t < ntiles - 1 ? t * TILE_W : last) + j + REF_WARP * i;
+}
+
 constexpr int RING = 256;        // ring slots
 constexpr int MIRROR = 80;       // slots [0, MIRROR) are also kept at RING + slot
 constexpr int RING_ALLOC = RING + MIRROR;
@@ -200,6 +240,9 @@ struct SweepIO {
   bool final_band;
   bool top_diag;     // lane 0 takes trow[x-1] as the diagonal of its first cell (resumed
                      // top row; needs no virtual rows in front of the band)
+  h2* lrow;          // FIX, final band: the last row also in the reference's tiled
+                     // layout (this read's row of KParams::last_row), or null
+  int32_t lr_ntiles, lr_last;
 };
 
 // Sweep one band across columns [x, c1) (+ ramp).  Accumulates the min/argmin
@@ -348,9 +391,13 @@ constexpr int RING2 = 512;    // slots; column offset o = c - x lives in slot (o
 constexpr int MIRROR2 = 136;  // slots [0, MIRROR2) are also kept at RING2 + slot
 constexpr int RING2_ALLOC = RING2 + MIRROR2;
 
-template <int S, int MODE>
+// QX (the reference's middle-tile flush index, below): on lanes with xdiag
+// set, the A column's rows k >= 2 take the next lane's B column row k - 2 of
+// this same step as their diagonal (made one iteration earlier and moved by
+// one DPP wave_shl:1) instead of their own previous column's row k - 1.
+template <int S, int MODE, bool QX = false>
 __device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 sB, h2 tinA, h2 tinB,
-                                           h2 diagA, h2 (&Bc)[S], h2& lastA) {
+                                           h2 diagA, h2 (&Bc)[S], h2& lastA, bool xdiag = false) {
   constexpr bool FUSED = MODE == DTWAX_F16_FUSED;
   // cA/cB: the pair's row costs t*t (per-op form) or t (fused form)
   const h2 t0 = hfma2(sA, B.A[0], B.Qn[0]);  // == rn(s - q) for a real row
@@ -361,7 +408,18 @@ __device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 
 #pragma unroll
   for (int k = 0; k <= S; k++) {
     h2 mA, mB, nA, nB;
-    if (k < S) mA = hmin2(a1, k == 0 ? diagA : Bc[k - 1]);  // A_k: top A_(k-1), diagonal Bprev_(k-1)
+    if constexpr (QX) {
+      if (k < S) {
+        h2 d = k == 0 ? diagA : Bc[k - 1];
+        if (k >= 2) {
+          const h2 nb = dpp_shl1(Bc[k - 1], Bc[k - 2]);  // next lane's B_(k-2), this step
+          d = xdiag ? nb : d;
+        }
+        mA = hmin2(a1, d);
+      }
+    } else {
+      if (k < S) mA = hmin2(a1, k == 0 ? diagA : Bc[k - 1]);  // A_k: top A_(k-1), diagonal Bprev_(k-1)
+    }
     if (k >= 1) mB = hmin2(b2, a2);                          // B_(k-1): top B_(k-2), diagonal A_(k-2)
     if (k + 1 < S) nA = hfma2(sA, B.A[k + 1], B.Qn[k + 1]);
     if (k < S) nB = hfma2(sB, B.A[k], B.Qn[k]);
@@ -384,12 +442,37 @@ __device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 
   lastA = a1;
 }
 
+// QK: the seam deviations of the reference kernel (src/include/DTW.cu), whose
+// schedule runs 2048 query rows as two passes of 1024 rows over tiles of 832
+// columns, one 32-lane warp with 26 columns per lane:
+//   Q1  row 1024 (the second pass's first row) takes +inf as its diagonal at
+//       columns 832k (k >= 1) and 832k + 26 (k >= 0): lane 0's diagonal is
+//       reset at every tile start (DTW.cu:315, 442) and lane 1's first
+//       diagonal is read before it is written (DTW.cu:179-180, 238-240);
+//   Q2  at columns 832k (k >= 2), rows 962..992 of either pass take
+//       D[r + 30][c - 1] as their diagonal: middle tiles flush the last
+//       column to penalty_last_col[wave - 63] (DTW.cu:385-388) where tile 0
+//       uses wave - 32 (DTW.cu:265-267).
+// Here the 2048 rows are one band (64 lanes x 32 rows, x = 0), so the
+// deviating cells are: lane 32 row 0 (Q1); lanes 30 and 62 rows 2..31 and
+// lanes 31 and 63 row 0 (Q2), all at even (A) columns.  Lane l reaches A
+// column 832k at step 416k + l, so the six deviating steps per tile are
+// t = 30, 31, 32, 45, 62, 63 (mod 416), which fall in four 4-step groups;
+// those groups run a slower step that applies them, the other 100 of every
+// 104 groups run the plain step.  Q2's source D[r + 30][c - 1] for lane 30
+// is lane 31's B column of the same step, made one iteration before lane 30
+// needs it (f16_cells2's QX form); for lane 31 it is its own previous B
+// column's row 30.
+constexpr int QK_PERIOD = TILE_W / 2;  // steps per tile (two columns per step)
+
 // One band over columns [x, c1) (+ ramp), two columns per step; band-final
 // min/argmin per strand over [c0, c1) into best/posf/posr, the band's last
 // row to io.orow.
-template <int S, int MODE>
+template <int S, int MODE, bool QK = false, bool FIX = false>
 __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const SweepIO& io, h2* ring_s,
                                           h2* ring_t, int lane, h2& best, int32_t& posf, int32_t& posr) {
+  static_assert(!QK || (S == 32 && MODE == DTWAX_F16 && FIX),
+                "seam deviations: binary16 per op, 64 x 32 rows, whole strands from column 0");
   const h2 INF = hsplat((half_t)__builtin_inff());
   const h2 ZERO = hsplat((half_t)0.0f);
   const int32_t x = io.x, c0 = io.c0, c1 = io.c1;
@@ -433,6 +516,7 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
   h2 sA = ring_s[base0], sB = ring_s[base0 + 1];
 
   const int32_t nsteps = (((c1 - x + 127) >> 1) + 63) & ~63;
+  int32_t tmod = 0;  // QK: T mod QK_PERIOD
   for (int32_t T = 0; T < nsteps; T += 64) {
     // offsets [2T + 130, 2T + 258): written to the ring after this block
     h2 pf_s[2], pf_t[2];
@@ -447,6 +531,35 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
     for (int g = 0; g < 64; g += 4) {
       const h2* rs = ring_s + base + 2 * g;
       const h2* rt = ring_t + base + 2 * g;
+      if constexpr (QK) {
+        int32_t r0 = tmod + g;
+        if (r0 >= QK_PERIOD) r0 -= QK_PERIOD;
+        if (r0 == 28 || r0 == 32 || r0 == 44 || r0 == 60) {
+          // a group holding a deviating step (uniform across the wave)
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int32_t t = T + g + u, r = r0 + u;
+            const bool q1 = (r == 32 && t >= QK_PERIOD + 32) || r == 45;      // lane 32
+            const bool q2 = (r == 30 || r == 31 || r == 62 || r == 63) && t >= 2 * QK_PERIOD + r;
+            h2 dA = diagA;
+            if (q1 && lane == 32) dA = INF;
+            if (q2 && (r & 1) && lane == r) dA = Bc[S - 2];                  // lanes 31, 63: own row 30
+            const bool xdiag = q2 && !(r & 1) && lane == r;                  // lanes 30, 62
+            const h2 nA = rs[2 * u + 2], nB = rs[2 * u + 3];
+            const h2 tfA = rt[2 * u + 2], tfB = rt[2 * u + 3];
+            h2 lastA;
+            f16_cells2<S, MODE, true>(B, sA, sB, tinA, tinB, dA, Bc, lastA, xdiag);
+            diagA = tinB;
+            tinA = dpp_shr1(tfA, lastA);
+            tinB = dpp_shr1(tfB, Bc[S - 1]);
+            histA = dpp_shl1(lastA, histA);
+            histB = dpp_shl1(Bc[S - 1], histB);
+            sA = nA;
+            sB = nB;
+          }
+          continue;
+        }
+      }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         // next step's columns; the top inputs of the next step (lane 0)
@@ -474,6 +587,9 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
         if (v.y < best.y) { best.y = v.y; posr = col; }
       }
       if (io.orow && col >= x && col < c1 && (!io.final_band || col >= c0)) io.orow[col] = v;
+      if constexpr (FIX)
+        if (io.lrow && io.final_band && col >= c0 && col < c1)
+          io.lrow[tiled_index(io.lr_ntiles, io.lr_last, col)] = v;
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -483,6 +599,10 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
       put(ring_t, slot, pf_t[h]);
     }
     __builtin_amdgcn_wave_barrier();
+    if constexpr (QK) {
+      tmod += 64;
+      if (tmod >= QK_PERIOD) tmod -= QK_PERIOD;
+    }
   }
 }
 
@@ -594,12 +714,14 @@ __device__ __forceinline__ void f16_band2_pack(const F16Band<S, MODE>& B, const 
 
 __device__ __forceinline__ half_t clamp_half(float v) { return (half_t)v; }
 
-template <int S, int MODE, bool M3>
+template <int S, int MODE, bool M3, bool QK = false, bool FIX = false>
 __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* ring_s, h2* ring_t,
                          h2* scratch) {
   const int32_t read = it.read;
-  const int32_t Q = p.q_len[read];
-  const float* __restrict__ qp = p.queries + p.q_off[read];
+  const int32_t Q = read_length<FIX>(p, read);
+  const int64_t qo = read_offset<FIX>(p, read);
+  const float* __restrict__ qp = p.queries + qo;
+  const uint32_t* __restrict__ qw = (const uint32_t*)p.queries + qo;  // half2 query words
   constexpr int BR = 64 * S;
   const int32_t nb = (Q + BR - 1) / BR;
   const int32_t P = nb * BR - Q;  // virtual rows in front of band 0
@@ -633,8 +755,12 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
       if (r >= 0) {
         // queries are read once: a non-temporal load keeps them from evicting
         // the reference, which every work item re-reads, from L2
-        half_t q = (half_t)__builtin_nontemporal_load(&qp[r]);
-        B.Qn[i] = hsplat(-q);
+        if constexpr (FIX) {
+          B.Qn[i] = -__builtin_bit_cast(h2, __builtin_nontemporal_load(&qw[r]));
+        } else {
+          half_t q = (half_t)__builtin_nontemporal_load(&qp[r]);
+          B.Qn[i] = hsplat(-q);
+        }
         B.A[i] = hsplat((half_t)1.0f);
       } else {
         B.Qn[i] = hsplat((half_t)0.0f);
@@ -652,11 +778,16 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     io.c0 = it.c0;
     io.c1 = it.c1;
     io.top_diag = (b == 0) && (trow_ext != nullptr);
+    if constexpr (FIX) {
+      io.lrow = p.last_row ? (h2*)p.last_row + (int64_t)read * p.lr_stride : nullptr;
+      io.lr_ntiles = p.lr_ntiles;
+      io.lr_last = p.lr_last;
+    }
     h2 rowmin = hsplat((half_t)__builtin_inff());
     h2 Kl = Kb;
     if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = Kb + prev_min;
     if constexpr (use_band2<S, MODE, M3>())
-      f16_band2<S, MODE>(B, io, ring_s, ring_t, lane, best, posf, posr);
+      f16_band2<S, MODE, QK, FIX>(B, io, ring_s, ring_t, lane, best, posf, posr);
     else
       f16_sweep<S, MODE, M3>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
     if (MODE == DTWAX_F16_RENORM) {
@@ -689,7 +820,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
   publish_min(&p.keys[2 * (int64_t)read + 1], wave_min_u64(kr), lane);
 }
 
-template <int S, int MODE, bool M3>
+template <int S, int MODE, bool M3, bool QK = false, bool FIX = false>
 __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
   constexpr int RA = use_band2<S, MODE, M3>() ? RING2_ALLOC : RING_ALLOC;
   __shared__ h2 ring_s[RA];
@@ -699,8 +830,8 @@ __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
   for (;;) {
     const int idx = fetch_item(p.counter, lane);
     if (idx >= p.n_items) break;
-    WorkItem it = p.items[idx];
-    f16_item<S, MODE, M3>(p, it, lane, ring_s, ring_t, scratch);
+    WorkItem it = load_item<FIX>(p, idx);
+    f16_item<S, MODE, M3, QK, FIX>(p, it, lane, ring_s, ring_t, scratch);
   }
 }
 
@@ -722,6 +853,8 @@ __device__ void f16_pack_item(const KParams& p, const WorkItem& it, int lane, h2
   io.c1 = it.c1;
   io.final_band = true;
   io.top_diag = false;
+  io.lrow = nullptr;  // packed plans never write the tiled last row
+  io.lr_ntiles = io.lr_last = 0;
   h2 best = INF;
   int32_t posf = -1, posr = -1;
   {
@@ -881,16 +1014,17 @@ __device__ __forceinline__ void f32_step(const float (&A)[S], const float (&Qn)[
   }
 }
 
-template <int S, int NSTR, bool M3>
+template <int S, int NSTR, bool M3, bool FIX = false>
 __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* ring_s,
                          float* ring_t, float* scratch) {
   const int32_t read = it.read;
-  const int32_t Q = p.q_len[read];
-  const float* __restrict__ qp = p.queries + p.q_off[read];
+  const int32_t Q = read_length<FIX>(p, read);
+  const float* __restrict__ qp = p.queries + read_offset<FIX>(p, read);
   constexpr int BR = 64 * S;
   const int32_t nb = (Q + BR - 1) / BR;
   const int32_t P = nb * BR - Q;
   const float INF = __builtin_inff();
+  float* lrow = (FIX && p.last_row) ? (float*)p.last_row + (int64_t)read * p.lr_stride * NSTR : nullptr;
   const float* __restrict__ ref = (const float*)p.ref;  // NSTR floats per column
   const float* trow_ext = nullptr;
   float* orow_ext = nullptr;
@@ -985,6 +1119,9 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
           pos[st] = col;
         }
         if (orow && in_item && (!final_band || col >= c0)) orow[(int64_t)col * NSTR + st] = hist[st];
+        if constexpr (FIX)
+          if (lrow && final_band && col >= c0 && col < c1)
+            lrow[tiled_index(p.lr_ntiles, p.lr_last, col) * NSTR + st] = hist[st];
       }
       __builtin_amdgcn_wave_barrier();
       const int slot = (T + 192 + lane) & (RING - 1);
@@ -1009,7 +1146,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
   }
 }
 
-template <int S, int NSTR, bool M3>
+template <int S, int NSTR, bool M3, bool FIX = false>
 __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
   __shared__ float ring_s[RING_ALLOC * NSTR];
   __shared__ float ring_t[RING_ALLOC * NSTR];
@@ -1018,8 +1155,8 @@ __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
   for (;;) {
     const int idx = fetch_item(p.counter, lane);
     if (idx >= p.n_items) break;
-    WorkItem it = p.items[idx];
-    f32_item<S, NSTR, M3>(p, it, lane, ring_s, ring_t, scratch);
+    WorkItem it = load_item<FIX>(p, idx);
+    f32_item<S, NSTR, M3, FIX>(p, it, lane, ring_s, ring_t, scratch);
   }
 }
 
@@ -1051,6 +1188,28 @@ __global__ void finalize_kernel(const unsigned long long* keys, int32_t n_reads,
     }
   }
   out[i] = r;
+}
+
+// keys -> the reference's dists layout (one half2 {fwd, rev} word or one
+// float per read); a read with no column scored gets +inf.
+__global__ void finalize_dists_kernel(const unsigned long long* keys, int32_t n_reads, int32_t h2out,
+                                      void* dists) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_reads) return;
+  float v[2];
+  for (int k = 0; k < 2; k++) {
+    const unsigned long long key = keys[2 * (int64_t)i + k];
+    const uint32_t hi = (uint32_t)(key >> 32);
+    const uint32_t u = (hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi;
+    v[k] = key == ~0ull ? __builtin_inff() : __builtin_bit_cast(float, u);
+  }
+  if (h2out) {
+    // binary16 scores are exact in float, so the conversion back is exact
+    const h2 w = (h2){(half_t)v[0], (half_t)v[1]};
+    ((uint32_t*)dists)[i] = __builtin_bit_cast(uint32_t, w);
+  } else {
+    ((float*)dists)[i] = v[0];
+  }
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t value) {
@@ -1122,6 +1281,18 @@ static hipError_t launch_f16_pack(const KParams& p, int S, int grid, hipStream_t
   return hipGetLastError();
 }
 
+hipError_t launch_sdtw_fixed(const KParams& p, int precision, int grid, bool quirks, hipStream_t st) {
+  if (precision == DTWAX_F16) {
+    if (quirks) hipLaunchKernelGGL((sdtw_f16_kernel<32, DTWAX_F16, false, true, true>), dim3(grid), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL((sdtw_f16_kernel<32, DTWAX_F16, false, false, true>), dim3(grid), dim3(64), 0, st, p);
+  } else if (precision == DTWAX_F32 && !quirks) {
+    hipLaunchKernelGGL((sdtw_f32_kernel<32, 1, false, true>), dim3(grid), dim3(64), 0, st, p);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_sdtw(const KParams& p, int precision, int S, int nstr, int grid, bool m3,
                        hipStream_t st, bool pack) {
   if (pack) {
@@ -1166,6 +1337,14 @@ hipError_t launch_finalize(const unsigned long long* keys, int32_t n_reads, int3
   return hipGetLastError();
 }
 
+hipError_t launch_finalize_dists(const unsigned long long* keys, int32_t n_reads, bool h2, void* dists,
+                                 hipStream_t st) {
+  if (n_reads <= 0) return hipSuccess;
+  hipLaunchKernelGGL(finalize_dists_kernel, dim3((n_reads + 255) / 256), dim3(256), 0, st, keys, n_reads,
+                     h2 ? 1 : 0, dists);
+  return hipGetLastError();
+}
+
 hipError_t launch_selftest_lanes(int32_t* d_out, hipStream_t st) {
   hipLaunchKernelGGL(selftest_lanes_kernel, dim3(1), dim3(64), 0, st, d_out);
   return hipGetLastError();
@@ -1201,6 +1380,17 @@ int occupancy_f16(int precision, int S, bool m3) {
   }
 #undef OCC_S
 #undef OCC
+  return n;
+}
+
+int occupancy_fixed(int precision, bool quirks) {
+  int n = 0;
+  if (precision == DTWAX_F32)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<32, 1, false, true>, 64, 0);
+  else if (quirks)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<32, DTWAX_F16, false, true, true>, 64, 0);
+  else
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<32, DTWAX_F16, false, false, true>, 64, 0);
   return n;
 }
 

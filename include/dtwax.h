@@ -10,7 +10,9 @@
  *
  * Reference interfaces each entry point replaces (paths relative to the
  * DTWax checkout):
- *   dtwax_distances        FullDTW::distances            src/include/DTW.hpp:285-295
+ *   dtwax_distances        FullDTW::distances (-DFP16)   src/include/DTW.hpp:285-295
+ *   dtwax_distances_f32    FullDTW::distances (FP32)     src/include/DTW.hpp:285-295,
+ *                                                         src/include/common.hpp:47-57
  *   dtwax_score            the batch loop + distances     src/main.cu:211-274
  *   dtwax_normalize        normalizer::normalize          src/include/normalizer.cu:71-114
  *   dtwax_reference_*      load_reference + tile upload   src/include/load_reference.hpp:33-108,
@@ -84,9 +86,8 @@ typedef struct {
                             the no-deletion recurrence */
   float renorm_drift;    /* DTWAX_F16_RENORM: initial per-row drift subtracted at lane
                             boundaries (adapted band to band); <= 0 = default 0.01 */
-  int32_t compat_quirks; /* 1: reproduce the reference kernel's band-seam and tile-seam
-                            deviations (DTW.cu:179-180, 315, 385-388, 442); needs the
-                            reference geometry (see dtwax_distances) */
+  int32_t compat_quirks; /* must be 0: the reference kernel's seam deviations exist only
+                            in its fixed geometry, reached through dtwax_distances */
   int32_t moves;         /* 2 (or 0): D = cost + min(top, diag), the shipped -DNO_REF_DEL
                             build; 3: + the left move D[r][c-1] (reference deletion, the
                             build without -DNO_REF_DEL, DTW.cu:23-27).  3 needs DTWAX_F16
@@ -288,18 +289,55 @@ int dtwax_autotune(dtwax_ctx* ctx, const float* d_queries, const int64_t* d_offs
                    dtwax_score_opts* best, dtwax_tune_entry* table, int32_t table_cap,
                    int32_t* n_table);
 
-/* Mirror of FullDTW::distances (src/include/DTW.hpp:285-295) for the
- * reference's fixed geometry (QUERY_LEN 2048, PREFIX_LEN 1024, SEGMENT 26,
- * WARP 32, REF_LEN 48128, -DFP16 -DNO_REF_DEL): ref = REF_LEN half2 {fwd, rev}
- * in the reference's tiled layout (src/main.cu:92-109); query = num_entries *
- * 2048 half2 {q, q} plus 32 pad entries; dists = num_entries half2 {fwd, rev}.
- * thresh is accepted and unused, as in the reference (DTW.cu:152-153);
- * device_last_row may be NULL (the scratch is internal).  compat_quirks
- * selects bit-exact reproduction of the reference kernel's seam deviations
- * (1) or the clean recurrence (0).  All pointers are device pointers. */
+/* Mirror of FullDTW::distances (src/include/DTW.hpp:285-295) in the
+ * reference's shipped build (-DFP16 -DNO_REF_DEL; src/include/common.hpp:
+ * 60-98: QUERY_LEN 2048, PREFIX_LEN 1024, SEGMENT 26, WARP 32, REF_LEN 48128,
+ * of which 57 tiles x 832 = 47,424 columns are scored).  Same buffers and
+ * layouts as the reference's launch (src/main.cu:92-121, 165-204):
+ *   ref    REF_LEN half2 {fwd, rev} words in the tiled layout of
+ *          src/main.cu:92-109 (h[l*832 + i*32 + j] = flat[l*832 + j*26 + i]);
+ *   query  num_entries * 2048 half2 words {q_fwd, q_rev} (the reference
+ *          passes {q, q}); its 32-word pad is accepted and never read;
+ *   dists  num_entries half2 words {fwd, rev}: min over the last DP row;
+ *   thresh accepted and unused, as in the reference (DTW.cu:152-153);
+ *   device_last_row  NULL, or num_entries * REF_LEN half2 words: on return
+ *          entry [e * REF_LEN + k] holds read e's last DP row (row 2047) at
+ *          the scored column whose tiled index is k (tile t < 56 at t*832,
+ *          tile 56 at REF_LEN - 832, as DTW.cu:293, 419, 546 store it);
+ *          entries of no scored column are left as they were.  The
+ *          reference also uses this buffer as its band hand-over scratch;
+ *          here that is internal and the buffer is output only.
+ * compat_quirks selects the DP: 1 = the reference kernel's own, including
+ * its two seam deviations (a +inf diagonal at row 1024, columns 832k and
+ * 832k + 26, DTW.cu:179-180, 315, 442; rows 962..992 of either 1024-row pass
+ * taking D[r+30][c-1] as their diagonal at columns 832k, k >= 2,
+ * DTW.cu:385-388), bit for bit, at the speed of the plain kernel; 0 = the
+ * clean recurrence D = (s-q)^2 + min(top, diag) with the same per-op
+ * binary16 rounding; 2 = the same output as 1 from a second, independent
+ * kernel that follows the reference's tile-by-tile schedule (one wave per
+ * read, about half the speed; a cross-check).
+ * The call is asynchronous on `stream` and allocates only on first use (the
+ * context's untiled-reference buffer, and per-read keys when num_entries
+ * grows); it does not touch the reference of dtwax_reference_upload.  Calls
+ * on one context share those buffers and must not overlap.  All pointers
+ * are device pointers. */
 int dtwax_distances(dtwax_ctx* ctx, const void* ref, const void* query, void* dists,
                     int32_t num_entries, uint32_t thresh, void* stream,
                     void* device_last_row, int32_t compat_quirks);
+
+/* The same mirror for the reference's FP32 build (FullDTW::distances
+ * instantiated with value_ht = float, common.hpp:47-57, 69-72): ref = REF_LEN
+ * = 96256 floats, the concatenation fwd[0:48128) | rev[0:48128) in the tiled
+ * layout; 115 tiles (95,680 columns) scored as ONE sequence (the DP crosses
+ * the fwd -> rev seam); query = num_entries * 2048 floats (+ pad); dists =
+ * num_entries floats; device_last_row = NULL or num_entries * REF_LEN floats
+ * (tile 114 at REF_LEN - 832).  Binary32 per op.  compat_quirks 1 and 2 both
+ * run the tile-by-tile kernel (the deviations are the same as in the FP16
+ * build: they come from the schedule, not the type); 0 the clean recurrence
+ * on the persistent kernel. */
+int dtwax_distances_f32(dtwax_ctx* ctx, const void* ref, const void* query, void* dists,
+                        int32_t num_entries, float thresh, void* stream,
+                        void* device_last_row, int32_t compat_quirks);
 
 /* ---- Read Until serving ---------------------------------------------------
  * Replaces the stateless loop of src/ru_main.cu:172-326 (10 FIFO queries of

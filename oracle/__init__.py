@@ -76,7 +76,8 @@ def lib():
         L.orc_cpu_baseline_f32.argtypes = [_f32p, _i64p, _i64p, C.c_int64, _f32p,
                                            C.c_int64, C.c_int, _f32p, _i64p, C.c_int]
         L.orc_emulate_reference.restype = C.c_int
-        L.orc_emulate_reference.argtypes = [C.c_void_p, _f32p, _f32p, C.c_int64, _f32p, C.c_int]
+        L.orc_emulate_reference.argtypes = [C.c_void_p, _f32p, _f32p, C.c_int64, _f32p, C.c_void_p,
+                                            C.c_int]
         L.orc_numpy_sum.restype = C.c_double
         L.orc_numpy_sum.argtypes = [_f64p, C.c_int64]
         L.orc_python_znorm.argtypes = [_f64p, C.c_int64, _f64p]
@@ -270,20 +271,28 @@ def cpu_baseline_f32(queries, strands, n_threads=0):
 
 def emulate_reference(ref, queries, query_len=2048, prefix_len=1024, segment=26,
                       warp=32, ref_len=48128, fp16=True, fix_q1=False, fix_q2=False,
-                      n_threads=0):
+                      n_threads=0, last_row=False):
     """Literal emulation of the reference kernel.  ref: fp16: (2, ref_len)
     fwd/rev values (already binary16-representable); fp32: (ref_len,).
-    queries: (n_reads, query_len) normalised floats."""
+    queries: (n_reads, query_len) normalised floats.  last_row=True also
+    returns device_last_row as the kernel leaves it: (n_reads, ref_len, 2)
+    for fp16, (n_reads, ref_len) for fp32, tiled index order, +inf where the
+    kernel never writes."""
     g = EmuGeom(query_len, prefix_len, segment, warp, ref_len, 1 if fp16 else 0,
                 1 if fix_q1 else 0, 1 if fix_q2 else 0)
     ref = np.ascontiguousarray(ref, np.float32).ravel()
     q = np.ascontiguousarray(queries, np.float32).ravel()
     n = q.size // query_len
     out = np.empty(n * (2 if fp16 else 1), np.float32)
-    rc = lib().orc_emulate_reference(C.addressof(g), ref, q, n, out, n_threads)
+    lr = np.empty(n * ref_len * (2 if fp16 else 1), np.float32) if last_row else None
+    rc = lib().orc_emulate_reference(C.addressof(g), ref, q, n, out,
+                                     None if lr is None else lr.ctypes.data, n_threads)
     if rc != 0:
         raise ValueError("unsupported emulator geometry")
-    return out.reshape(n, 2) if fp16 else out
+    res = out.reshape(n, 2) if fp16 else out
+    if last_row:
+        return res, (lr.reshape(n, ref_len, 2) if fp16 else lr.reshape(n, ref_len))
+    return res
 
 
 # ------------------------------------------------------------ normalisation

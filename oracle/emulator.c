@@ -151,7 +151,7 @@ static val2 query_at(const blk_in* b, int64_t idx) {
   return vset(0.0f); /* past the 32-entry pad: never consumed */
 }
 
-static void emulate_block(const emu_ctx* e, const blk_in* b, float* out) {
+static void emulate_block(const emu_ctx* e, const blk_in* b, float* out, float* last_row) {
   const orc_emu_geom* g = e->g;
   const int W = g->warp, SEG = g->segment, PREFIX = g->prefix_len;
   const int TILE = SEG * W;
@@ -300,6 +300,11 @@ static void emulate_block(const emu_ctx* e, const blk_in* b, float* out) {
   }
   out[0] = L[W - 1].min_segment.v[0];
   if (e->nc == 2) out[1] = L[W - 1].min_segment.v[1];
+  /* device_last_row after the kernel (the last pass's rows; entries the
+   * reference never writes keep their +inf initialisation here) */
+  if (last_row)
+    for (int i = 0; i < g->ref_len; i++)
+      for (int k = 0; k < e->nc; k++) last_row[(int64_t)i * e->nc + k] = dlr[i].v[k];
   free(plc);
   free(dlr);
   free(L);
@@ -307,7 +312,7 @@ static void emulate_block(const emu_ctx* e, const blk_in* b, float* out) {
 
 int orc_emulate_reference(const orc_emu_geom* g, const float* ref,
                           const float* queries, int64_t n_reads, float* out,
-                          int n_threads) {
+                          float* last_row, int n_threads) {
   if (g->warp < 1 || g->warp > MAXW || (g->warp & (g->warp - 1))) return -1;
   if (g->segment < 1 || g->segment > MAXSEG) return -1;
   if (g->prefix_len < 1 || g->query_len % g->prefix_len) return -1;
@@ -339,7 +344,8 @@ int orc_emulate_reference(const orc_emu_geom* g, const float* ref,
     b.ref = ref;
     b.query = qv + r * g->query_len;
     b.query_avail = total - r * g->query_len;
-    emulate_block(&e, &b, out + r * e.nc);
+    emulate_block(&e, &b, out + r * e.nc,
+                  last_row ? last_row + r * (int64_t)g->ref_len * e.nc : NULL);
   }
   free(qv);
   return 0;

@@ -132,10 +132,13 @@ typedef struct {
 /* ref: fp16: 2*ref_len halves as float values (fwd[ref_len] then rev[ref_len]);
  *      fp32: ref_len floats.  queries: n_reads*query_len normalised floats.
  * out: fp16: n_reads*2 (fwd, rev) half values as float; fp32: n_reads floats.
+ * last_row (optional): n_reads*ref_len entries (2 floats each for fp16) of
+ * device_last_row as the kernel leaves it (tiled index order; entries it
+ * never writes are +inf).
  * Returns 0, or -1 on an unsupported geometry.                             */
 int orc_emulate_reference(const orc_emu_geom* g, const float* ref,
                           const float* queries, int64_t n_reads, float* out,
-                          int n_threads);
+                          float* last_row, int n_threads);
 
 /* ---- statistics and normalisation -------------------------------------- */
 /* numpy's pairwise summation order for a float64 reduction (what np.sum /

@@ -49,3 +49,49 @@ def golden_reference_f64():
         return O.python_znorm(v[:48128])
 
     return np.concatenate([sig(fa), sig(rev_comp_python(fa))])
+
+
+# ---- the reference's launch layouts (src/main.cu:92-109, 165-179) ----------
+REF_TILE, REF_SEG, REF_WARP = 832, 26, 32
+
+
+def tile_reference(words):
+    """Flat 32-bit words (REF_LEN,) -> the reference's tiled layout
+    h[l*832 + i*32 + j] = flat[l*832 + j*26 + i] over REF_LEN // 832 tiles;
+    the tail beyond them is never copied (left 0)."""
+    words = np.ascontiguousarray(words)
+    n_tiles = words.shape[0] // REF_TILE
+    c = np.arange(n_tiles * REF_TILE)
+    l, rem = c // REF_TILE, c % REF_TILE
+    out = np.zeros_like(words)
+    out[l * REF_TILE + (rem % REF_SEG) * REF_WARP + rem // REF_SEG] = words[c]
+    return out
+
+
+def tiled_index(cols, ref_len):
+    """Index of each column in the reference's tiled last-row layout (the
+    last tile at ref_len - 832, src/include/DTW.cu:451, 546)."""
+    cols = np.asarray(cols)
+    n_tiles = ref_len // REF_TILE
+    t, w = cols // REF_TILE, cols % REF_TILE
+    base = np.where(t < n_tiles - 1, t * REF_TILE, ref_len - REF_TILE)
+    return base + w // REF_SEG + REF_WARP * (w % REF_SEG)
+
+
+def half2_words(fwd, rev):
+    """Two float arrays (binary16-representable or not: rounded RN) -> uint32
+    words {fwd, rev} as __floats2half2_rn packs them."""
+    a = np.asarray(fwd, np.float32).astype(np.float16).view(np.uint16).astype(np.uint32)
+    b = np.asarray(rev, np.float32).astype(np.float16).view(np.uint16).astype(np.uint32)
+    return a | (b << 16)
+
+
+def query_words_h2(qs):
+    """(n, 2048) float queries -> n*2048 + 32 half2 words {q, q} (zero pad)."""
+    w = half2_words(qs.ravel(), qs.ravel())
+    return np.concatenate([w, np.zeros(32, np.uint32)])
+
+
+def words_to_h2_floats(words):
+    """uint32 half2 words -> (n, 2) float32 (fwd, rev)."""
+    return np.ascontiguousarray(words, np.uint32).view(np.float16).reshape(-1, 2).astype(np.float32)

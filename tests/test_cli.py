@@ -50,11 +50,27 @@ def test_usage_and_errors():
     assert p.returncode == 1 and "Error opening" in p.stderr
 
 
+def _lambda_squiggle(n=48128):
+    _, means = load_model_means()
+    fa = fasta_python()
+    rc = fa[::-1].translate(str.maketrans("ACGT", "TGCA"))
+    lv, _ = O.squiggle_reference(fa, rc, means, n)
+    return lv
+
+
+def _test_read_query():
+    sig = np.load(os.path.join(GOLDEN, "batch0_signal.npy")).astype(np.float32)[1952:4000]
+    q, _, _ = O.norm_meanstd(sig, 1e-4)
+    return q
+
+
 @pytest.mark.gpu
 def test_table_matches_oracle():
-    """Full run on the test read: the table rows carry the binary16 scores of
-    the clean recurrence over the 47,424 scored columns of the 48,128-k-mer
-    squiggle (mean/std normalised query, eps 1e-4)."""
+    """Full run on the test read: by default the table rows carry the
+    reference kernel's own binary16 scores (the literal emulator of DTW.cu,
+    seam deviations included) over the 47,424 scored columns of the
+    48,128-k-mer squiggle (mean/std normalised query, eps 1e-4); with --clean
+    the clean recurrence's."""
     p = run()
     lines = p.stdout.splitlines()
     hdr = "Read_ID\tQUERY_LEN\tREF_LEN\tsDTW score: fwd-strand\tsDTW score: rev-strand"
@@ -64,19 +80,19 @@ def test_table_matches_oracle():
     assert len(rows) == 1 and rows[0][1] == "read_0083a4a2-b062-44ed-988e-e7f5a301121f"
     assert rows[0][2] == "2048" and rows[0][3] == "48128"
     fwd, rev = float(rows[0][4]), float(rows[0][5])
-    # oracle
-    _, means = load_model_means()
-    fa = fasta_python()
-    rc = fa[::-1].translate(str.maketrans("ACGT", "TGCA"))
-    lv, _ = O.squiggle_reference(fa, rc, means, 48128)
-    sig = np.load(os.path.join(GOLDEN, "batch0_signal.npy")).astype(np.float32)[1952:4000]
-    q, _, _ = O.norm_meanstd(sig, 1e-4)
+    lv = _lambda_squiggle()
+    q = _test_read_query()
+    flat = O.f16_to_f32(O.f32_to_f16(lv.ravel())).reshape(2, -1)
+    want = O.emulate_reference(flat, q[None, :])[0]
+    assert float(f"{want[0]:g}") == fwd and float(f"{want[1]:g}") == rev, (fwd, rev, want)
+    # --clean: the clean recurrence on the same launch
+    p = run("--clean")
+    row = [l.split("\t") for l in p.stdout.splitlines() if re.match(r"^\d+\tread_", l)][0]
     qh = O.f32_to_f16(q)
-    want = []
     for k in range(2):
         sc, _ = O.sdtw_f16(qh, O.f32_to_f16(lv[k, :47424]))
-        want.append(float(O.f16_to_f32(np.array([sc], np.uint16))[0]))
-    assert float(f"{want[0]:g}") == fwd and float(f"{want[1]:g}") == rev, (fwd, rev, want)
+        w = float(O.f16_to_f32(np.array([sc], np.uint16))[0])
+        assert float(f"{w:g}") == float(row[4 + k]), (k, row, w)
 
 
 @pytest.mark.gpu
@@ -84,7 +100,8 @@ def test_read_until_fifo_matches_oracle(tmp_path):
     """The Read Until form (src/ru_main.cu:172-326): 10 x 2048 bytes per batch
     from the FIFO `pipe` in the working directory, one unsigned sample per
     byte, mean/std normalised, rows `j j 2048 48128 fwd rev` without a
-    header; the loop ends when the writer closes the FIFO."""
+    header; the loop ends when the writer closes the FIFO.  Scores are the
+    reference kernel's own (the same kernel as the offline driver)."""
     import errno
     import time
     fifo = tmp_path / "pipe"
@@ -108,28 +125,32 @@ def test_read_until_fifo_matches_oracle(tmp_path):
     assert p.returncode == 0, err
     rows = [l.split("\t") for l in out.splitlines() if l.strip()]
     assert len(rows) == 20 and all(len(r) == 6 for r in rows)
-    _, means = load_model_means()
-    fa = fasta_python()
-    rc = fa[::-1].translate(str.maketrans("ACGT", "TGCA"))
-    lv, _ = O.squiggle_reference(fa, rc, means, 48128)
-    sh = [O.f32_to_f16(lv[k, :47424]) for k in range(2)]
+    lv = _lambda_squiggle()
+    flat = O.f16_to_f32(O.f32_to_f16(lv.ravel())).reshape(2, -1)
+    check = [0, 1, 2, 10, 11, 12]  # rows of both batches, through the emulator
+    qs = []
+    for i in check:
+        b, j = divmod(i, 10)
+        sig = data[b * 20480 + j * 2048:b * 20480 + (j + 1) * 2048].astype(np.float32)
+        q, _, _ = O.norm_meanstd(sig, 1e-4)
+        qs.append(q)
+    want = O.emulate_reference(flat, np.stack(qs))
     for i, r in enumerate(rows):
         b, j = divmod(i, 10)
         assert r[:4] == [str(j), str(j), "2048", "48128"]
-        sig = data[b * 20480 + j * 2048:b * 20480 + (j + 1) * 2048].astype(np.float32)
-        q, _, _ = O.norm_meanstd(sig, 1e-4)
-        qh = O.f32_to_f16(q)
-        for k in range(2):
-            sc, _ = O.sdtw_f16(qh, sh[k])
-            want = float(O.f16_to_f32(np.array([sc], np.uint16))[0])
-            assert float(f"{want:g}") == float(r[4 + k]), (i, k, r, want)
+        if i in check:
+            w = want[check.index(i)]
+            for k in range(2):
+                assert float(f"{w[k]:g}") == float(r[4 + k]), (i, k, r, w)
 
 
 @pytest.mark.gpu
 def test_fp32_build_and_native_modes():
     """--fp32-build: the reference's FP32 build (one concatenated fwd|rev
-    sequence of 2 x 48,128 levels, 115 tiles of 832 scored, one score column);
-    --native --precision f32 --norm medmad: every k-mer, both strands, binary32."""
+    sequence of 2 x 48,128 levels, 115 tiles of 832 scored, one score column)
+    with its kernel's own DP (the emulator in its FP32 mode), and with
+    --clean the clean binary32 recurrence; --native --precision f32 --norm
+    medmad: every k-mer, both strands, binary32."""
     _, means = load_model_means()
     fa = fasta_python()
     rc = fa[::-1].translate(str.maketrans("ACGT", "TGCA"))
@@ -141,9 +162,13 @@ def test_fp32_build_and_native_modes():
     row = [l.split("\t") for l in lines if re.match(r"^\d+\tread_", l)][0]
     assert row[2] == "2048" and row[3] == "96256" and len(row) == 5
     lv, _ = O.squiggle_reference(fa, rc, means, 48128)
-    cat = np.concatenate([lv[0], lv[1]])[: (2 * 48128 // 832) * 832]
+    full = np.concatenate([lv[0], lv[1]])
     q, _, _ = O.norm_meanstd(sig, 1e-4)
-    want, _ = O.sdtw_f32(q, cat)
+    want = O.emulate_reference(full, q[None, :], ref_len=2 * 48128, fp16=False)[0]
+    assert float(f"{want:g}") == float(row[4]), (row, want)
+    p = run("--fp32-build", "--clean")
+    row = [l.split("\t") for l in p.stdout.splitlines() if re.match(r"^\d+\tread_", l)][0]
+    want, _ = O.sdtw_f32(q, full[: (2 * 48128 // 832) * 832])
     assert float(f"{want:g}") == float(row[4]), (row, want)
     # native, binary32, median/MAD
     p = run("--native", "--precision", "f32", "--norm", "medmad")
