@@ -78,6 +78,7 @@ def lib():
             "dtwax_plan_kernel_ms": ([vp, C.POINTER(f32)], C.c_int),
             "dtwax_distances": ([vp, vp, vp, vp, i32, C.c_uint32, vp, vp, i32], C.c_int),
             "dtwax_distances_f32": ([vp, vp, vp, vp, i32, f32, vp, vp, i32], C.c_int),
+            "dtwax_distances_noalloc": ([vp, vp, vp, vp, i32, C.c_uint32, vp, vp, i32], C.c_int),
             "dtwax_selftest_lanes": ([vp], C.c_int),
             "dtwax_autotune": ([vp, vp, vp, vp, i32, vp, i32, i64, C.c_char_p, vp, vp, i32,
                                 C.POINTER(i32)], C.c_int),
@@ -117,7 +118,8 @@ def exported_symbols():
         "dtwax_event_destroy", "dtwax_event_record", "dtwax_event_elapsed_ms",
         "dtwax_reference_upload", "dtwax_squiggle_build", "dtwax_normalize", "dtwax_score",
         "dtwax_plan_create", "dtwax_plan_destroy", "dtwax_plan_info", "dtwax_score_plan",
-        "dtwax_plan_set_timing", "dtwax_plan_packing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_distances_f32", "dtwax_selftest_lanes",
+        "dtwax_plan_set_timing", "dtwax_plan_packing", "dtwax_plan_kernel_ms", "dtwax_distances", "dtwax_distances_f32",
+        "dtwax_distances_noalloc", "dtwax_selftest_lanes",
         "dtwax_autotune", "dtwax_ru_create", "dtwax_ru_destroy", "dtwax_ru_info", "dtwax_ru_stats", "dtwax_ru_process",
         "dtwax_ru_release", "dtwax_ru_start", "dtwax_ru_submit", "dtwax_ru_poll", "dtwax_ru_stop",
         "dtwax_vbz_decode", "dtwax_pod5_open", "dtwax_pod5_close", "dtwax_pod5_count",

@@ -40,6 +40,28 @@ int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 #define HIP_TRY(expr) DTWAX_HIP_TRY(expr)
 
+// ---- scratch of the allocation-free entry point -----------------------------
+// dtwax_distances_noalloc keeps everything beyond its inputs and outputs in
+// this file-scope device array, so no call allocates: the untiled, padded
+// reference (FP16 build: 64 + 47,424 + 384 half2 words), one work counter
+// and 2 keys per read for at most kNoAllocMaxEntries reads, 448 KB per GPU
+// for the life of the process.  Host code never uses the name `scratch`
+// itself (on the host it is only a stand-in for the symbol): each device's
+// address comes from hipGetSymbolAddress once, in dtwax_create with that
+// device current, and is kept per device.
+namespace {
+constexpr int32_t kNoAllocMaxEntries = 16384;
+constexpr size_t kNoAllocRefWords = COL_PADL + 47424 + COL_PADR;
+constexpr size_t kNoAllocRefBytes = (kNoAllocRefWords * 4 + 255) / 256 * 256;
+constexpr size_t kNoAllocCounterBytes = 256;
+constexpr size_t kNoAllocKeyBytes = (size_t)kNoAllocMaxEntries * 2 * sizeof(unsigned long long);
+constexpr size_t kNoAllocBytes = kNoAllocRefBytes + kNoAllocCounterBytes + kNoAllocKeyBytes;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 
 extern "C" {
 
@@ -54,6 +76,15 @@ int dtwax_create(int device, dtwax_ctx** out) {
   HIP_TRY(hipSetDevice(device));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (device < kMaxDevices) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void* ptr = nullptr;
+      HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      if (!ptr) return fail(DTWAX_EDEVICE, "hipGetSymbolAddress(scratch) gave no address");
+      g_scratch_ptr[device] = ptr;
+    }
+  }
   auto* c = new dtwax_ctx;
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
@@ -905,29 +936,26 @@ struct RefBuild {
 constexpr RefBuild kBuildF16{true, 48128, 48128 / TILE_W, (48128 / TILE_W) * TILE_W};
 constexpr RefBuild kBuildF32{false, 96256, 96256 / TILE_W, (96256 / TILE_W) * TILE_W};
 
-int distances_impl(dtwax_ctx* c, const RefBuild& g, const void* ref, const void* query, void* dists,
-                   int32_t num_entries, void* stream, void* device_last_row, int32_t compat_quirks) {
+uint32_t pad_word(const RefBuild& g) {
+  const uint16_t ph = __builtin_bit_cast(uint16_t, kPadF16);
+  return g.fp16 ? ((uint32_t)ph | ((uint32_t)ph << 16)) : __builtin_bit_cast(uint32_t, kPadF32);
+}
+
+int check_distances_args(dtwax_ctx* c, const void* ref, const void* query, void* dists, int32_t num_entries,
+                         int32_t compat_quirks) {
   if (!c || !ref || !query || !dists) return fail(DTWAX_EINVAL, "NULL argument");
   if (num_entries < 0) return fail(DTWAX_EINVAL, "num_entries < 0");
   if (compat_quirks < 0 || compat_quirks > 2) return fail(DTWAX_EINVAL, "compat_quirks must be 0, 1 or 2");
-  if (num_entries == 0) return DTWAX_OK;
-  HIP_TRY(hipSetDevice(c->device));
-  hipStream_t st = S_(stream);
-  // The untiled reference in the context's own buffer for this build (the
-  // reference uploaded with dtwax_reference_upload is left alone), padded
-  // like it so the sweep reads past both ends without bounds checks.
-  void*& rbuf = c->compat_ref[g.fp16 ? 0 : 1];
-  if (!rbuf) {
-    const int64_t total = COL_PADL + g.n_cols + COL_PADR;
-    if (hipMalloc(&rbuf, (size_t)total * 4) != hipSuccess) {
-      rbuf = nullptr;
-      return fail(DTWAX_ENOMEM, "reference-layout reference buffer");
-    }
-    const uint16_t ph = __builtin_bit_cast(uint16_t, kPadF16);
-    const uint32_t pad = g.fp16 ? ((uint32_t)ph | ((uint32_t)ph << 16)) : __builtin_bit_cast(uint32_t, kPadF32);
-    HIP_TRY(launch_fill_u32((uint32_t*)rbuf, total, pad, st));
-  }
-  uint32_t* flat = (uint32_t*)rbuf + COL_PADL;
+  return DTWAX_OK;
+}
+
+// One reference-layout batch on the given buffers: rbuf = the padded
+// reference buffer (pads already set), counter/keys for num_entries reads.
+// Kernels only: nothing here allocates.
+int distances_run(dtwax_ctx* c, const RefBuild& g, uint32_t* rbuf, int32_t* counter, unsigned long long* keys,
+                  const void* ref, const void* query, void* dists, int32_t num_entries, hipStream_t st,
+                  void* device_last_row, int32_t compat_quirks) {
+  uint32_t* flat = rbuf + COL_PADL;
   HIP_TRY(launch_untile_reference((const uint32_t*)ref, g.n_tiles, flat, st));
   const int32_t lr_last = g.ref_len - TILE_W;
   if (compat_quirks == 2 || (compat_quirks == 1 && !g.fp16)) {
@@ -936,17 +964,6 @@ int distances_impl(dtwax_ctx* c, const RefBuild& g, const void* ref, const void*
     HIP_TRY(launch_compat_tiles(g.fp16, flat, query, num_entries, g.n_tiles, dists, device_last_row, g.ref_len,
                                 lr_last, st));
     return DTWAX_OK;
-  }
-  if (c->compat_cap < num_entries) {
-    if (c->compat_keys) (void)hipFree(c->compat_keys);
-    if (c->compat_counter) (void)hipFree(c->compat_counter);
-    c->compat_keys = nullptr;
-    c->compat_counter = nullptr;
-    c->compat_cap = 0;
-    if (hipMalloc(&c->compat_keys, (size_t)num_entries * 2 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&c->compat_counter, 256) != hipSuccess)
-      return fail(DTWAX_ENOMEM, "reference-layout keys");
-    c->compat_cap = num_entries;
   }
   const bool quirks = compat_quirks == 1;
   const int occ = occupancy_fixed(g.fp16 ? DTWAX_F16 : DTWAX_F32, quirks);
@@ -960,8 +977,8 @@ int distances_impl(dtwax_ctx* c, const RefBuild& g, const void* ref, const void*
   L.q_stride = kRefQuery;
   L.n_items = num_entries;
   L.n_reads = num_entries;
-  L.counter = c->compat_counter;
-  L.keys = c->compat_keys;
+  L.counter = counter;
+  L.keys = keys;
   L.ref = flat;
   L.n_cols = g.n_cols;
   L.quirks = quirks;
@@ -972,6 +989,40 @@ int distances_impl(dtwax_ctx* c, const RefBuild& g, const void* ref, const void*
   L.dists = dists;
   L.dists_h2 = g.fp16;
   return run_score_launch(c, L, st);
+}
+
+int distances_impl(dtwax_ctx* c, const RefBuild& g, const void* ref, const void* query, void* dists,
+                   int32_t num_entries, void* stream, void* device_last_row, int32_t compat_quirks) {
+  int rc = check_distances_args(c, ref, query, dists, num_entries, compat_quirks);
+  if (rc || num_entries == 0) return rc;
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t st = S_(stream);
+  // The untiled reference in the context's own buffer for this build (the
+  // reference uploaded with dtwax_reference_upload is left alone), padded
+  // like it so the sweep reads past both ends without bounds checks.
+  void*& rbuf = c->compat_ref[g.fp16 ? 0 : 1];
+  if (!rbuf) {
+    const int64_t total = COL_PADL + g.n_cols + COL_PADR;
+    if (hipMalloc(&rbuf, (size_t)total * 4) != hipSuccess) {
+      rbuf = nullptr;
+      return fail(DTWAX_ENOMEM, "reference-layout reference buffer");
+    }
+    HIP_TRY(launch_fill_u32((uint32_t*)rbuf, total, pad_word(g), st));
+  }
+  const bool keyed = !(compat_quirks == 2 || (compat_quirks == 1 && !g.fp16));
+  if (keyed && c->compat_cap < num_entries) {
+    if (c->compat_keys) (void)hipFree(c->compat_keys);
+    if (c->compat_counter) (void)hipFree(c->compat_counter);
+    c->compat_keys = nullptr;
+    c->compat_counter = nullptr;
+    c->compat_cap = 0;
+    if (hipMalloc(&c->compat_keys, (size_t)num_entries * 2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->compat_counter, 256) != hipSuccess)
+      return fail(DTWAX_ENOMEM, "reference-layout keys");
+    c->compat_cap = num_entries;
+  }
+  return distances_run(c, g, (uint32_t*)rbuf, c->compat_counter, c->compat_keys, ref, query, dists, num_entries,
+                       st, device_last_row, compat_quirks);
 }
 }  // namespace
 
@@ -985,6 +1036,32 @@ int dtwax_distances_f32(dtwax_ctx* c, const void* ref, const void* query, void* 
                         float thresh, void* stream, void* device_last_row, int32_t compat_quirks) {
   (void)thresh;
   return distances_impl(c, kBuildF32, ref, query, dists, num_entries, stream, device_last_row, compat_quirks);
+}
+
+int dtwax_distances_noalloc(dtwax_ctx* c, const void* ref, const void* query, void* dists, int32_t num_entries,
+                            uint32_t thresh, void* stream, void* device_last_row, int32_t compat_quirks) {
+  (void)thresh;
+  int rc = check_distances_args(c, ref, query, dists, num_entries, compat_quirks);
+  if (rc || num_entries == 0) return rc;
+  if (num_entries > kNoAllocMaxEntries)
+    return fail(DTWAX_EINVAL, "num_entries " + std::to_string(num_entries) +
+                                  " exceeds the allocation-free limit of " + std::to_string(kNoAllocMaxEntries));
+  void* base = nullptr;
+  if (c->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[c->device];
+  }
+  if (!base) return fail(DTWAX_ENOTREADY, "no allocation-free scratch on this device");
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t st = S_(stream);
+  uint32_t* rbuf = (uint32_t*)base;
+  int32_t* counter = (int32_t*)((unsigned char*)base + kNoAllocRefBytes);
+  unsigned long long* keys = (unsigned long long*)((unsigned char*)base + kNoAllocRefBytes + kNoAllocCounterBytes);
+  // the pads may not hold the pad value yet (the array starts zeroed): set
+  // the whole buffer, the interior is untiled over it next
+  HIP_TRY(launch_fill_u32(rbuf, (int64_t)kNoAllocRefWords, pad_word(kBuildF16), st));
+  return distances_run(c, kBuildF16, rbuf, counter, keys, ref, query, dists, num_entries, st, device_last_row,
+                       compat_quirks);
 }
 
 // ---- self test ---------------------------------------------------------------

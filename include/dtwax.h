@@ -339,6 +339,21 @@ int dtwax_distances_f32(dtwax_ctx* ctx, const void* ref, const void* query, void
                         int32_t num_entries, float thresh, void* stream,
                         void* device_last_row, int32_t compat_quirks);
 
+/* Allocation-free variant of dtwax_distances (same arguments, layouts,
+ * compat_quirks forms and results).  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture or a tight serving loop.  It calls no hipMalloc or hipFree:
+ * its scratch (the untiled reference, a work counter and 2 keys per read) is
+ * one file-scope device array of 448 KB on each GPU the library runs on,
+ * held until the process ends, whose address is taken when a context is
+ * created on that GPU.  At most 16384 entries per call (DTWAX_EINVAL
+ * beyond).  All calls on one GPU share that array, which keeps the last
+ * call's contents (each call resets what it needs): calls that overlap on
+ * one GPU, on any streams or contexts, race. */
+int dtwax_distances_noalloc(dtwax_ctx* ctx, const void* ref, const void* query, void* dists,
+                            int32_t num_entries, uint32_t thresh, void* stream,
+                            void* device_last_row, int32_t compat_quirks);
+
 /* ---- Read Until serving ---------------------------------------------------
  * Replaces the stateless loop of src/ru_main.cu:172-326 (10 FIFO queries of
  * 2048 samples scored from scratch per iteration, scores printed, no
