@@ -354,6 +354,17 @@ bool parse_arrow(const uint8_t* f, size_t n, ArrowFile& af, std::vector<std::uni
         return false;
       b.bufs.push_back(sp);
     }
+    // Every row or element of a column takes at least one bit of its
+    // (decompressed) buffers -- a validity bit, a value or an offset -- so no
+    // batch or node length may exceed their bits.  This also keeps every
+    // later size product (length x width) far below 2^63.
+    uint64_t buf_bytes = 0;
+    for (const Span& sp : b.bufs) buf_bytes += (uint64_t)sp.n;
+    const int64_t max_len = (int64_t)std::min<uint64_t>(buf_bytes, (uint64_t)1 << 56) * 8;
+    if (b.length > max_len) return e.set("Arrow record batch longer than its buffers");
+    for (uint32_t k = 0; k < nn; k++)
+      if (b.node_len[k] > max_len || b.node_nulls[k] > b.node_len[k])
+        return e.set("Arrow node longer than its record batch buffers");
     af.batches.push_back(std::move(b));
   }
   return true;
@@ -420,13 +431,14 @@ bool scalars(const ArrowFile& af, const Batch& b, int col, Scalars& s, Err& e) {
   const int nd = af.node0[(size_t)col], bf = af.buf0[(size_t)col];
   if (b.node_len[(size_t)nd] != b.length) return e.set("column '" + f.name + "' length differs from its batch");
   const Span& v = b.bufs[(size_t)bf + 1];
-  if (v.n < b.length * bytes) return e.set("column '" + f.name + "' shorter than its batch");
+  if (b.length > v.n / bytes) return e.set("column '" + f.name + "' shorter than its batch");
   s.p = v.p;
   s.bytes = bytes;
   s.kind = kind;
   s.valid = b.bufs[(size_t)bf];
   s.nulls = b.node_nulls[(size_t)nd];
-  if (s.nulls && s.valid.n * 8 < b.length) return e.set("column '" + f.name + "': validity bitmap too short");
+  if (s.nulls && b.length / 8 + (b.length % 8 != 0) > s.valid.n)
+    return e.set("column '" + f.name + "': validity bitmap too short");
   return true;
 }
 
@@ -463,11 +475,12 @@ bool lists(const ArrowFile& af, const Batch& b, int col, Lists& L, Err& e, bool 
     L.values = b.bufs[(size_t)bf + 3];
     L.value_bytes = f.kids[0].width / 8;
     L.n_values = b.node_len[(size_t)nd + 1];
-    if (L.values.n < L.n_values * L.value_bytes) return e.set("column '" + f.name + "': values shorter than announced");
+    if (L.n_values > L.values.n / L.value_bytes)
+      return e.set("column '" + f.name + "': values shorter than announced");
   }
   const Span& o = b.bufs[(size_t)bf + 1];
   const int ob = L.off64 ? 8 : 4;
-  if (b.length > 0 && o.n < (b.length + 1) * ob) return e.set("column '" + f.name + "': offsets too short");
+  if (b.length > 0 && b.length >= o.n / ob) return e.set("column '" + f.name + "': offsets too short");
   L.off = o.p;
   int64_t prev = b.length > 0 ? L.begin(0) : 0;
   if (prev < 0) return e.set("column '" + f.name + "': negative offset");
@@ -629,7 +642,7 @@ bool open_pod5(dtwax_pod5& P, Err& e) {
   P.row_begin.push_back(0);
   for (const Batch& b : reads.batches) {
     const int nd = reads.node0[(size_t)c_id], bf = reads.buf0[(size_t)c_id];
-    if (b.node_len[(size_t)nd] != b.length || b.node_nulls[(size_t)nd] || b.bufs[(size_t)bf + 1].n < 16 * b.length)
+    if (b.node_len[(size_t)nd] != b.length || b.node_nulls[(size_t)nd] || b.length > b.bufs[(size_t)bf + 1].n / 16)
       return e.set("POD5 read_id column malformed or null");
     const uint8_t* ids = b.bufs[(size_t)bf + 1].p;
     Lists rl;
@@ -728,15 +741,18 @@ extern "C" int dtwax_pod5_signal(const dtwax_pod5* f, const int64_t* reads, int6
   using dtwax::set_error;
   if (!f || (n > 0 && (!reads || !n_out))) return set_error(DTWAX_EINVAL, "NULL argument");
   if (n < 0 || skip < 0) return set_error(DTWAX_EINVAL, "negative count or skip");
+  if (take > INT64_MAX - skip) return set_error(DTWAX_EINVAL, "skip + take overflows");
   const int64_t nr = (int64_t)f->info.size();
   std::vector<int64_t> base((size_t)n + 1, 0);
   for (int64_t j = 0; j < n; j++) {
     if (reads[j] < 0 || reads[j] >= nr) return set_error(DTWAX_EINVAL, "read index outside the file");
     const int64_t have = std::max<int64_t>(0, f->info[(size_t)reads[j]].n_samples - skip);
-    base[(size_t)j + 1] = base[(size_t)j] + (take >= 0 ? take : have);
+    const int64_t add = take >= 0 ? take : have;
+    // base[j] <= dst_cap here, so this cannot overflow; a larger total is refused
+    if (add > dst_cap - base[(size_t)j]) return set_error(DTWAX_EINVAL, "destination too small");
+    base[(size_t)j + 1] = base[(size_t)j] + add;
   }
   if (base[(size_t)n] > 0 && !dst) return set_error(DTWAX_EINVAL, "NULL destination");
-  if (base[(size_t)n] > dst_cap) return set_error(DTWAX_EINVAL, "destination too small");
   int threads = n_threads > 0 ? n_threads : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
   threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
   std::atomic<int64_t> next{0};

@@ -234,6 +234,63 @@ def test_refusals(tmp_path):
             f.signal(reads=[0])
 
 
+_OPEN_AND_READ = r"""
+import sys
+sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
+import dtwax_amd as D
+try:
+    with D.Pod5File({path!r}) as f:
+        f.signal()
+    print("read")
+except D.DtwaxError as e:
+    print("error:", e)
+"""
+
+
+def test_oversized_lengths_refused(tmp_path):
+    """Lengths read from the file cannot overflow the size checks: a plain
+    (non-VBZ) file whose signal child node claims 2^62 values and whose row
+    offsets claim 2^40 is refused at open (before, it opened and the copy of
+    the row crashed the process).  Run in a child so a crash fails the test."""
+    import struct
+    p = tmp_path / "ok.pod5"
+    write_pod5(p, [make_read(np.arange(1000, dtype=np.int16))], vbz=False)
+    raw = open(p, "rb").read()
+    node = struct.pack("<qq", 1000, 0)  # the signal list's child node: 1000 values, no nulls
+    # the row's large_list offsets [0, 1000], followed by the samples 0, 1, 2, 3
+    offs = struct.pack("<qq", 0, 1000) + struct.pack("<4h", 0, 1, 2, 3)
+    assert raw.count(node) == 1 and raw.count(offs) == 1
+    big_node = struct.pack("<qq", 1 << 62, 0)
+    big_offs = struct.pack("<qq", 0, 1 << 40) + struct.pack("<4h", 0, 1, 2, 3)
+    cases = {"node": raw.replace(node, big_node), "offsets": raw.replace(offs, big_offs),
+             "both": raw.replace(node, big_node).replace(offs, big_offs)}
+    for name, data in cases.items():
+        q = tmp_path / f"{name}.pod5"
+        q.write_bytes(data)
+        code = _OPEN_AND_READ.format(root=ROOT, tests=os.path.dirname(__file__), path=str(q))
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (name, r.returncode, r.stderr[-2000:])
+        assert r.stdout.startswith("error:"), (name, r.stdout)
+
+
+def test_signal_window_overflow_refused(tmp_path):
+    """dtwax_pod5_signal refuses windows whose sizes overflow (a huge take
+    would wrap the destination offsets below the capacity check)."""
+    p = tmp_path / "w.pod5"
+    write_pod5(p, _reads([3000, 2000]))
+    with D.Pod5File(p) as f:
+        idx = np.array([0, 1], np.int64)
+        n_out = np.zeros(2, np.int64)
+        dst = np.zeros(16, np.int16)
+        big = (1 << 62) + 8
+        rc = D.lib().dtwax_pod5_signal(f._h, idx.ctypes.data, 2, 0, big, dst.ctypes.data, 16,
+                                       n_out.ctypes.data, 1)
+        assert rc == -1 and b"too small" in D.lib().dtwax_last_error()
+        rc = D.lib().dtwax_pod5_signal(f._h, idx.ctypes.data, 2, (1 << 63) - 100, 200, dst.ctypes.data, 16,
+                                       n_out.ctypes.data, 1)
+        assert rc == -1 and b"overflow" in D.lib().dtwax_last_error()
+
+
 _FUZZ = r"""
 import sys, numpy as np
 sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
