@@ -8,8 +8,9 @@ them statelessly and never answers).  This module is the load generator for
 the stateful engine instead:
 
 * n_channels channels sample at sample_rate Hz; reads start after an
-  exponential gap (mean gap_s) and have exponential lengths (mean
-  read_mean samples, at least one chunk); target_frac of them come from the
+  exponential gap (mean gap_s; with burst, every channel's first read
+  starts at t = 0) and have exponential lengths (mean read_mean samples,
+  at least one chunk); target_frac of them come from the
   reference (random strand and position), the rest from unrelated sequence;
 * every chunk_len samples a read delivers one chunk, until it is decided;
   a NONTARGET read is unblocked (ends unblock_s after the decision), a
@@ -56,6 +57,7 @@ class SimConfig:
     unblock_s: float = 0.1
     target_frac: float = 0.1
     seed: int = 0
+    burst: bool = False  # every channel starts its first read at t = 0 (else after a gap)
 
 
 class ReadSource:
@@ -148,7 +150,7 @@ class FlowCell:
         self.number = np.zeros(cfg.n_channels, np.int64)
         self.decided = []
         for ch in range(cfg.n_channels):
-            self.push(float(self.rng.exponential(cfg.gap_s)), "start", ch)
+            self.push(0.0 if cfg.burst else float(self.rng.exponential(cfg.gap_s)), "start", ch)
 
     def push(self, t, kind, ch, payload=None):
         heapq.heappush(self.events, (t, self.seq, kind, ch, payload))

@@ -95,3 +95,17 @@ def query_words_h2(qs):
 def words_to_h2_floats(words):
     """uint32 half2 words -> (n, 2) float32 (fwd, rev)."""
     return np.ascontiguousarray(words, np.uint32).view(np.float16).reshape(-1, 2).astype(np.float32)
+
+
+def norm_chunks_medmad(raw, chunk_len):
+    """The Read Until engine's median/MAD normalisation: the first chunk's
+    statistics applied to every chunk, y = (float)((x - centre) * inv) in
+    binary64 (inv = 1 / (1.4826 * MAD), 0 for a constant chunk)."""
+    import oracle as O
+    first = raw[:chunk_len]
+    y0, med, mad = O.norm_medmad(first)
+    scale = 1.4826 * mad
+    inv = 1.0 / scale if scale > 0 else 0.0
+    y = ((np.asarray(raw, np.float64) - med) * inv).astype(np.float32)
+    assert np.array_equal(y[:chunk_len], y0)
+    return y

@@ -291,6 +291,23 @@ def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     assert np.array_equal(sc, sc0) and np.array_equal(am, am0)
 
 
+def test_autotune_with_packing_forced(ctx):
+    """packing = 1 leaves the widths without a packed kernel (8, 12) out of
+    the candidates instead of failing on them; the choice scores exactly."""
+    rng = np.random.default_rng(78)
+    ref = rng.standard_normal((2, 20000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    lens = rng.integers(300, 5000, 40)
+    reads = [rng.standard_normal(int(n)).astype(np.float32) for n in lens]
+    flat, offs, lns = D.pack_reads(reads)
+    d_q, d_off = D.DeviceArray.from_host(flat), D.DeviceArray.from_host(offs)
+    best, table = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, packing=1)
+    assert sorted(table["segment"].tolist()) == [0, 16, 20, 25, 28, 32]
+    sc, am = ctx.score(reads, precision="f16", packing=1, **best)
+    osc, oam = _oracle_f16(reads, ref)
+    assert np.array_equal(sc, osc) and np.array_equal(am, oam)
+
+
 def test_plan_and_score_argument_errors(ctx):
     """Bad options fail with a status and a message, not a fault."""
     rng = np.random.default_rng(4)

@@ -10,10 +10,13 @@ number of decisions that flip against FP64 at a fixed, length-scaled
 threshold (per-row threshold = midpoint of the FP64 classes at Q = 2000,
 times Q).
 
-The default test runs a 48-read subset at Q = 1000, 2000 and 8000 and checks the
-expected ordering of the forms.  With DTWAX_FULL_SWEEP=1 the full sweep
-(1024 reads x Q in {1000, 2000, 4000, 8000}) runs and writes its report to
-DTWAX_SWEEP_OUT (default precision_sweep.json).
+The test runs the full sweep (1024 reads x Q in {1000, 2000, 4000, 8000},
+about 20 s with the FP64 oracle on 16 host threads) and checks at every
+length: binary32 within 1e-4 relative of FP64 with no flips; the
+renormalised form within 2% (mean 0.5%) with at most one flip (measured: 1 at
+Q = 1000 against binary16's 22, 0 at the other lengths); plain binary16
+drifting low with the query length.  With DTWAX_SWEEP_OUT set it writes its
+report there.
 """
 import json
 import os
@@ -85,25 +88,21 @@ def sweep(ctx, n_reads, lengths, seed=3, threads=0):
             "threshold_per_row": float(thr_row), "lengths": {q: per_q[q] for q in sorted(per_q)}}
 
 
-def test_precision_subset(ctx):
-    rep = sweep(ctx, 48, (1000, 2000, 8000))
-    print(json.dumps(rep))
-    for Q, row in rep["lengths"].items():
-        f16, rn, f32 = row["f16"], row["f16_renorm"], row["f32"]
-        assert f32["max_rel"] < 1e-4, (Q, f32)
-        assert f32["flips"] == 0
-        assert rn["max_rel"] < 0.02 and rn["mean_rel"] < 0.005, (Q, rn)
-        assert rn["mean_rel"] <= f16["mean_rel"], (Q, rn, f16)
-        assert rn["flips"] <= f16["flips"] + 1
-    # binary16 cost accumulation drifts with the query length
-    assert rep["lengths"][8000]["f16"]["mean_rel"] > rep["lengths"][1000]["f16"]["mean_rel"]
-
-
-@pytest.mark.skipif(not os.environ.get("DTWAX_FULL_SWEEP"), reason="full sweep: set DTWAX_FULL_SWEEP=1")
-def test_precision_full_sweep(ctx):
+def test_precision_sweep(ctx):
     t0 = time.time()
     rep = sweep(ctx, 1024, (1000, 2000, 4000, 8000))
     rep["wall_s"] = round(time.time() - t0, 1)
-    with open(os.environ.get("DTWAX_SWEEP_OUT", "precision_sweep.json"), "w") as f:
-        json.dump(rep, f, indent=1)
+    if os.environ.get("DTWAX_SWEEP_OUT"):
+        with open(os.environ["DTWAX_SWEEP_OUT"], "w") as f:
+            json.dump(rep, f, indent=1)
     print(json.dumps(rep))
+    assert sorted(rep["lengths"]) == [1000, 2000, 4000, 8000]
+    for Q, row in rep["lengths"].items():
+        f16, rn, f32 = row["f16"], row["f16_renorm"], row["f32"]
+        assert f32["max_rel"] < 1e-4 and f32["flips"] == 0, (Q, f32)
+        assert rn["max_rel"] < 0.02 and rn["mean_rel"] < 0.005, (Q, rn)
+        assert rn["flips"] <= 1 and rn["flips"] <= f16["flips"], (Q, rn, f16)
+        assert rn["mean_rel"] <= f16["mean_rel"], (Q, rn, f16)
+    # binary16 cost accumulation drifts low with the query length
+    means = [rep["lengths"][q]["f16"]["mean_rel"] for q in (1000, 2000, 4000, 8000)]
+    assert means == sorted(means) and means[-1] > 0.1

@@ -16,6 +16,7 @@ import pytest
 import dtwax_amd as D
 import oracle as O
 from dtwax_amd import synth
+from helpers import norm_chunks_medmad
 
 pytestmark = pytest.mark.gpu
 
@@ -34,15 +35,8 @@ def _workload(seed, n_reads, chunk_len, n_chunks, ref_bp=8_000):
 
 
 def _norm_chunks(raw, chunk_len):
-    """Median/MAD statistics of the first chunk applied to every chunk, as
-    y = (float)((x - centre) * inv) in binary64."""
-    first = raw[:chunk_len]
-    y0, med, mad = O.norm_medmad(first)
-    scale = 1.4826 * mad
-    inv = 1.0 / scale if scale > 0 else 0.0
-    y = ((raw.astype(np.float64) - med) * inv).astype(np.float32)
-    assert np.array_equal(y[:chunk_len], y0)
-    return y
+    """Median/MAD statistics of the first chunk applied to every chunk."""
+    return norm_chunks_medmad(raw, chunk_len)
 
 
 def _oracle_chunked(y, ref, chunk_len, precision):
