@@ -181,6 +181,9 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
   // 0: per-read widths (the plan's own choice)
   std::vector<int32_t> segs = (o.precision == DTWAX_F16_RENORM) ? std::vector<int32_t>{8, 16}
                                                                  : std::vector<int32_t>{0, 8, 12, 16, 20, 25, 28, 32};
+  // packing = 1 packs at the widths that have a packed kernel only (automatic, 16..32)
+  if (o.packing == 1) segs.erase(std::remove_if(segs.begin(), segs.end(), [](int32_t S) { return S && S < 16; }),
+                                 segs.end());
   if (o.segment) segs = {o.segment};
   // Replicated sample: the full workload has far more work items than the
   // sample, so a candidate is timed on enough copies of the sample reads to
