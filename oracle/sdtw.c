@@ -18,9 +18,15 @@
  *    matrix and the final min 286.8895631) -- tests/test_oracle_golden.py;
  *    full-matrix brute force on random tiny inputs; closed forms (Q = 1,
  *    exact substring => 0) -- tests/test_oracle_sdtw.py.
- *  - orc_sdtw_f32 / _f16 / _f16_fused: a per-cell reimplementation from the
- *    brute-force matrix with numpy float32 / float16 scalar arithmetic on
- *    tiny inputs, and agreement with FP64 to the precision's error bound.
+ *  - orc_sdtw_f32 / _f16: a per-cell restatement with numpy float32 /
+ *    float16 scalar arithmetic on tiny inputs (every last-row cell), and
+ *    agreement with FP64 to the precision's error bound.
+ *  - orc_sdtw_f16_fused: a per-cell restatement in exact rationals (t*t + m
+ *    formed exactly, one round-to-nearest-even to binary16 decided by exact
+ *    comparison with the neighbouring binary16 values) on tiny inputs, every
+ *    last-row cell; on 2 x 1 inputs where rounding t*t first changes the
+ *    result, the fused and per-op forms differ and each equals its own
+ *    restatement (tests/test_oracle_sdtw.py).
  *  - orc_cpu_baseline_f32: bit-equal to orc_sdtw_f32.
  */
 #include <float.h>

@@ -5,7 +5,7 @@ import pytest
 import oracle as O
 
 
-def _np_matrix(q, s, dt, fused=False):
+def _np_matrix(q, s, dt):
     """Per-cell restatement with numpy scalars of dtype dt (independent of the
     C code): D[r][c] = rn(rn(rn(s-q)^2) + min(D[r-1][c], D[r-1][c-1]))."""
     Q, N = len(q), len(s)
@@ -98,7 +98,87 @@ def test_f16_vector_path_equals_scalar():
             assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2])
 
 
-def test_f16_fused_close_and_distinct():
+def _rn16_exact(x):
+    """Binary16 round-to-nearest-even of an exact rational x >= 0 (value
+    comparison against the neighbouring binary16 values in exact arithmetic,
+    so no intermediate rounding)."""
+    from fractions import Fraction
+    if x >= Fraction(65520):  # halfway between 65504 and the next binade: inf
+        return np.float16(np.inf)
+    h = np.float16(float(x))  # nearest or a neighbour of it
+    with np.errstate(over="ignore"):  # the neighbour above 65504 is inf, dropped below
+        cands = {h, np.nextafter(h, np.float16(np.inf)), np.nextafter(h, np.float16(0))}
+    cands = [c for c in cands if np.isfinite(c) and c >= 0]
+    best = min(abs(Fraction(float(c)) - x) for c in cands)
+    tied = [c for c in cands if abs(Fraction(float(c)) - x) == best]
+    if len(tied) == 1:
+        return tied[0]
+    return [c for c in tied if int(c.view(np.uint16)) % 2 == 0][0]
+
+
+def _fused_matrix_exact(qh, sh):
+    """Per-cell restatement of the fused binary16 form with exact rationals:
+    t = rn16(s - q); D = rn16(t*t + min(top, diag)), the product and the sum
+    exact, one rounding per cell."""
+    from fractions import Fraction
+    Q, N = len(qh), len(sh)
+    D = np.zeros((Q, N), np.float16)
+    inf = np.float16(np.inf)
+    for r in range(Q):
+        for c in range(N):
+            t = np.float16(sh[c] - qh[r])  # binary16 subtraction, exactly rounded
+            top = np.float16(0) if r == 0 else D[r - 1, c]
+            dg = np.float16(0) if r == 0 else (D[r - 1, c - 1] if c > 0 else inf)
+            m = min(top, dg)
+            if not np.isfinite(m):
+                D[r, c] = inf
+                continue
+            tf = Fraction(float(t))
+            D[r, c] = _rn16_exact(tf * tf + Fraction(float(m)))
+    return D
+
+
+def test_rn16_exact_rounding():
+    from fractions import Fraction
+    for v in (0.0, 1.0, 65504.0, 6e-8, 0.1, 1000.3):
+        assert _rn16_exact(Fraction(v)) == np.float16(v)
+    one = Fraction(1)
+    assert _rn16_exact(one + Fraction(1, 2 ** 11)) == np.float16(1.0)  # tie -> even
+    assert _rn16_exact(one + Fraction(3, 2 ** 11)) == np.float16(1 + 2 ** -9)  # tie -> even (up)
+    assert _rn16_exact(one + Fraction(1, 2 ** 11) + Fraction(1, 2 ** 40)) == np.float16(1 + 2 ** -10)
+    assert np.isinf(_rn16_exact(Fraction(65520)))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_f16_fused_per_cell_exact(seed):
+    """orc_sdtw_f16_fused equals the exact-rational per-cell restatement of
+    D = rn16(t*t + min) (one rounding), on every cell of the last row."""
+    rng = np.random.default_rng(200 + seed)
+    Q, N = 12, 20
+    qh = (rng.standard_normal(Q) * 2).astype(np.float16)
+    sh = (rng.standard_normal(N) * 2).astype(np.float16)
+    D = _fused_matrix_exact(qh, sh)
+    sc, am, row = O.sdtw_f16(qh.view(np.uint16), sh.view(np.uint16), want_row=True, fused=True)
+    assert np.array_equal(row, D[-1].view(np.uint16))
+    assert am == int(np.argmin(D[-1]))
+
+
+def test_f16_fused_and_per_op_differ_where_rounding_twice_matters():
+    """Two rows, one column: D1 = t1^2 + D0 rounded once (fused) or after
+    rounding t1^2 first (per op, the reference's COST_FUNCTION); for these
+    values the two differ, and each form equals its own restatement."""
+    for qv, sv in (((0.1257, -0.1321), 0.6406), ((1.304, 0.9473), -0.7036)):
+        qh = np.array(qv, np.float16)
+        sh = np.array([sv], np.float16)
+        per_op = _np_matrix(qh, sh, np.float16)
+        fused = _fused_matrix_exact(qh, sh)
+        assert per_op[-1, 0] != fused[-1, 0]
+        a, _ = O.sdtw_f16(qh.view(np.uint16), sh.view(np.uint16))
+        b, _ = O.sdtw_f16(qh.view(np.uint16), sh.view(np.uint16), fused=True)
+        assert a == per_op[-1].view(np.uint16)[0] and b == fused[-1].view(np.uint16)[0]
+
+
+def test_f16_fused_close_to_fp64():
     rng = np.random.default_rng(7)
     q = O.f32_to_f16(rng.standard_normal(400))
     s = O.f32_to_f16(rng.standard_normal(2000))
