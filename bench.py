@@ -14,7 +14,18 @@ length x counted columns x 2 strands); `roofline` prices the sDTW kernel's
 own launch time (HIP events recorded on its stream) against the packed-VALU
 peak N_CU x 64 lanes x 2 halves x f_clk, at 3 element ops per cell.
 `cpu_baseline` times the FP32 CPU restatement (all host cores, OpenMP over
-reads, SIMD over columns) on a bounded sample of the same reads.
+reads, SIMD over columns, column strips; built for the host's best ISA) on a
+bounded sample of the same reads.
+
+Beside the headline the default run also reports, each with its own
+figures: the other arithmetic forms (`forms`); the reference's own launch
+through dtwax_distances at its shipped geometry, with and without its seam
+deviations (`reference_kernel`); one autotuned pass of the uneven workload
+(16,384 reads of U[500, 8000] samples vs a 5 Mb reference; `uneven_5mb`,
+with its own roofline and CPU baseline); and Read Until serving at 512
+channels against the 5 Mb reference (`read_until`: a saturated batch, 60 s
+of a served flow cell, the plain batched form, and a burst with every
+channel starting at t = 0).
 """
 import argparse
 import json
@@ -51,8 +62,11 @@ def parse():
                     help="target CPU time of the bounded cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ru", action="store_true", help="skip the Read Until serving leg")
-    ap.add_argument("--ru-seconds", type=float, default=10.0,
+    ap.add_argument("--no-c4", action="store_true", help="skip the one-pass uneven 5 Mb leg")
+    ap.add_argument("--ru-seconds", type=float, default=60.0,
                     help="simulated flow-cell seconds of the Read Until leg")
+    ap.add_argument("--burst-seconds", type=float, default=10.0,
+                    help="simulated seconds of the Read Until burst (every channel starts at t = 0)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="also run one plain device copy of 256 MiB (normaliser copy mode) "
                          "before timing, as the byte-count calibration for HBM counters")
@@ -60,6 +74,11 @@ def parse():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sdtw_f16_kernel_48kb_4096x2000_traffic.json")
+
+
+TRAFFIC_SOURCE = ("profiles/sdtw_f16_kernel_48kb_4096x2000_traffic.md: rocprofv3 --pmc FETCH_SIZE "
+                  "and WRITE_SIZE of this launch, each scaled by a plain copy kernel's reading; "
+                  "a committed measurement, not taken in this run")
 
 
 def measured_traffic():
@@ -72,7 +91,7 @@ def measured_traffic():
         return None
 
 
-def read_until_leg(ctx, seconds, norm="meanstd"):
+def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
     """Read Until serving on the 5 Mb workload: 512 channels, 1600-sample
     chunks (0.4 s at 4 kHz), decisions within 4 chunks, FP16, median/MAD of
     the first chunk ("meanstd": the reference's own mean/std on both the
@@ -113,6 +132,12 @@ def read_until_leg(ctx, seconds, norm="meanstd"):
         cfg = ru_sim.SimConfig(n_channels=512, duration_s=seconds, seed=1)
         m = ru_sim.simulate_realtime(ru, src, cfg)
         print(f"[bench] read until: served {m['sim_seconds']} s", file=sys.stderr, flush=True)
+        if burst_seconds > 0:
+            for c in range(512):
+                ru.release(c)
+            bcfg = ru_sim.SimConfig(n_channels=512, duration_s=burst_seconds, seed=2, burst=True)
+            b = ru_sim.simulate_realtime(ru, src, bcfg)
+            print(f"[bench] read until: burst {b['sim_seconds']} s", file=sys.stderr, flush=True)
     finally:
         ru.close()
     pthr, pcal = ru_sim.calibrate_plain_threshold(ctx, src, 4 * 1600, n_each=128, norm=mode)
@@ -133,6 +158,12 @@ def read_until_leg(ctx, seconds, norm="meanstd"):
         "serving_vs_plain_reads_per_s": round(m["reads_decided_per_s"] / max(plain["reads_decided_per_s"], 1e-9), 3),
         "setup_s": round(time.time() - t0, 1),
     })
+    if burst_seconds > 0:
+        m["burst"] = {"what": "every channel's first read starts at t = 0, so all 512 first chunks arrive "
+                              "together (the saturated batch), then Poisson as above",
+                      **{k: b[k] for k in ("sim_seconds", "reads_started", "reads_decided", "reads_decided_per_s",
+                                           "latency_ms_p50", "latency_ms_p99", "latency_ms_max",
+                                           "decision_latency_ms_p99", "mean_batch", "accuracy")}}
     return m
 
 
@@ -154,22 +185,27 @@ def cpu_sample_rate(qs, strands, cpu_seconds):
     return ccells / tc, int(used), n, tc
 
 
-def main_c4(a):
+def host_description():
+    import oracle as O
+    model, _ = O.cpu_flags()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "isa": O.baseline_lib()[1]}
+
+
+def uneven_leg(a, steps, warmup, ctx=None):
     """16,384 reads, lengths uniform in [500, 8000] samples, 10% target, vs
     a 5 Mb random reference (2 x 4,999,995 columns), FP16, median/MAD.  The
     autotuner picks segment and reference chunk on a 512-read x 2^20-column
-    sample; one step = one persistent launch over every (read, chunk) item."""
+    sample; one step = one persistent launch over every (read, chunk) item.
+    Returns the result object (the JSON line of --workload c4)."""
     import ctypes as C
     import dtwax_amd as D
     from dtwax_amd import synth
-    steps = 1 if a.steps is None else a.steps
-    warmup = 0 if a.warmup is None else a.warmup
     t0 = time.time()
     w = synth.config_workload(seed=4, ref_bp=5_000_000, n_reads=16384, raw_len=None,
                               len_range=(500, 8000), target_frac=0.1)
     gen_s = time.time() - t0
     print(f"[bench] c4 workload generated in {gen_s:.1f} s", file=sys.stderr, flush=True)
-    ctx = D.Context(0)
+    ctx = ctx or D.Context(0)
     info = ctx.device_info()
     ctx.upload_reference(w.ref_levels)
     n_cols = w.ref_levels.shape[1]
@@ -189,10 +225,10 @@ def main_c4(a):
         best, table = dict(segment=a.segment, ref_chunk=a.ref_chunk), []
     else:
         best, table = ctx.autotune(d_norm, d_off, lens, sample_reads=512, sample_cols=1 << 20,
-                                   cache_path=a.tune_cache, precision=a.precision)
+                                   cache_path=a.tune_cache, precision="f16")
     tune_s = time.time() - t1
     print(f"[bench] c4 autotune {best} in {tune_s:.1f} s", file=sys.stderr, flush=True)
-    plan = ctx.plan(lens, precision=a.precision, packing=a.packing, **best)
+    plan = ctx.plan(lens, precision="f16", packing=a.packing, **best)
     pinfo = plan.info()
     plan.set_timing(True)
     for _ in range(warmup):
@@ -224,7 +260,7 @@ def main_c4(a):
     out = {
         "metric": METRIC, "value": round(cells / (step_ms * 1e-3) / 1e9, 1), "unit": "GCUPS",
         "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(step_ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": a.precision,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16",
         "data": "synthetic (seeded random 5 Mb reference, 6-mer model, simulated reads)",
         "config": {"workload": "16,384 reads x U[500, 8000] samples vs 5 Mb reference, both strands, "
                                "10% target, median/MAD, autotuned",
@@ -236,7 +272,8 @@ def main_c4(a):
         "kernel_ms": round(km, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 2), "peak": round(peak_ops / 1e12, 2),
                      "unit": "TOP/s", "frac": round(achieved / peak_ops, 4), "traffic": None,
-                     "ops_per_cell": 3, "computed_cells_over_counted": round(pinfo["computed"] / cells, 4)},
+                     "ops_per_cell": 3, "computed_cells_over_counted": round(pinfo["computed"] / cells, 4),
+                     "algorithmic_bytes": int(lens.sum()) * 4 + 2 * int(n_cols) * 2 + len(lens) * 48},
         "autotune": {"seconds": round(tune_s, 1), "choice": best,
                      "table": [{"segment": int(t["segment"]), "ref_chunk": int(t["ref_chunk"]),
                                 "sample_gcups": round(float(t["gcups"]), 1)} for t in table]},
@@ -253,8 +290,86 @@ def main_c4(a):
                                "sample": f"{n_sample} random reads of the 16,384 (of "
                                          f"{int(sum(lens[pick[:n_sample]]))} samples) vs the whole "
                                          f"reference, FP32, {tc:.1f} s on {used} threads; the full "
-                                         f"workload would take {cells / rate:.0f} s"}
-    print(json.dumps(out))
+                                         f"workload would take {cells / rate:.0f} s",
+                               **host_description()}
+    plan.close()
+    for arr in (d_raw, d_off, d_len, d_norm, d_out):
+        arr.free()
+    return out
+
+
+def main_c4(a):
+    steps = 1 if a.steps is None else a.steps
+    warmup = 0 if a.warmup is None else a.warmup
+    print(json.dumps(uneven_leg(a, steps, warmup)))
+
+
+def reference_kernel_leg(ctx, raw_reads, ref_levels, stream, peak_ops):
+    """The reference's own launch at its shipped geometry through the
+    FullDTW::distances mirror: 2048-sample queries (samples [952, 3000) of
+    the headline's 3000-sample reads, mean/std with eps 1e-4 as
+    normalizer.cu:86-100), the first 48,128 k-mers per strand as the tiled
+    half2 reference (57 tiles = 47,424 columns scored), half2 dists.  Timed
+    with HIP events around each call (untiling, keys, sDTW kernel, dists):
+    compat_quirks 1 (the reference kernel's DP, seam deviations included,
+    in the persistent kernel), 0 (the clean recurrence), 2 (the deviations
+    from the tile-by-tile kernel)."""
+    import ctypes as C
+    import dtwax_amd as D
+    L = D.lib()
+    ref_len, tile, seg, warp = 48128, 832, 26, 32
+    n_tiles = ref_len // tile
+    normed, _ = ctx.normalize([r[952:3000] for r in raw_reads], mode=D.NORM_MEANSTD, eps=1e-4,
+                              stream=stream)
+    qs = np.stack(normed).astype(np.float32)
+    n = qs.shape[0]
+
+    def h2(x, y):
+        a = np.asarray(x, np.float32).astype(np.float16).view(np.uint16).astype(np.uint32)
+        b = np.asarray(y, np.float32).astype(np.float16).view(np.uint16).astype(np.uint32)
+        return a | (b << 16)
+    flat = h2(ref_levels[0, :ref_len], ref_levels[1, :ref_len])
+    c = np.arange(n_tiles * tile)
+    tiled = np.zeros(ref_len, np.uint32)
+    tiled[(c // tile) * tile + (c % tile % seg) * warp + (c % tile) // seg] = flat[c]
+    words = np.concatenate([h2(qs.ravel(), qs.ravel()), np.zeros(32, np.uint32)])
+    d_ref, d_q = D.DeviceArray.from_host(tiled), D.DeviceArray.from_host(words)
+    d_out = D.DeviceArray((n,), np.uint32)
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    D._check(L.dtwax_event_create(C.byref(ev0)))
+    D._check(L.dtwax_event_create(C.byref(ev1)))
+    cells = n * 2048 * n_tiles * tile * 2
+    out, scores = {}, {}
+    for name, quirks in (("reference_dp", 1), ("clean", 0), ("reference_dp_tiles", 2)):
+        ms = []
+        for _ in range(2):  # the first call warms up
+            D._check(L.dtwax_event_record(ev0, stream))
+            ctx.distances(d_ref, d_q, d_out, n, compat_quirks=quirks, stream=stream)
+            D._check(L.dtwax_event_record(ev1, stream))
+            D._check(L.dtwax_stream_sync(stream))
+            v = C.c_float()
+            D._check(L.dtwax_event_elapsed_ms(ev0, ev1, C.byref(v)))
+            ms.append(v.value)
+        scores[quirks] = d_out.to_host().view(np.float16).reshape(n, 2).astype(np.float32)
+        out[name] = {"compat_quirks": quirks, "call_ms": round(ms[-1], 3),
+                     "gcups": round(cells / (ms[-1] * 1e-3) / 1e9, 1),
+                     "roofline_frac": round(3.0 * cells / (ms[-1] * 1e-3) / peak_ops, 4)}
+    L.dtwax_event_destroy(ev0)
+    L.dtwax_event_destroy(ev1)
+    d = scores[1] - scores[0]
+    best_r, best_c = scores[1].min(axis=1), scores[0].min(axis=1)
+    cuts = np.unique(best_r)
+    out.update({
+        "workload": f"{n} reads x 2048 samples (mean/std, eps 1e-4) vs the tiled half2 reference of "
+                    "48,128 k-mers per strand, 47,424 columns scored, both strands",
+        "tiles_equal_persistent": bool(np.array_equal(scores[1], scores[2])),
+        "clean_vs_reference_dp": {"changed_strand_scores": int(np.count_nonzero(d)),
+                                  "max_abs_diff": float(np.abs(d).max()),
+                                  "mean_abs_diff": float(f"{np.abs(d).mean():.4g}"),
+                                  "max_decision_flips_over_all_thresholds":
+                                      int(max(np.sum((best_r < t) != (best_c < t)) for t in cuts))},
+    })
+    return out
 
 
 def main():
@@ -372,6 +487,7 @@ def main():
                      "peak": round(peak_ops / 1e12, 2), "unit": "TOP/s",
                      "frac": round(achieved_ops / peak_ops, 4),
                      "traffic": (measured_traffic() or {}).get("hbm_bytes_per_launch"),
+                     "traffic_source": TRAFFIC_SOURCE,
                      "algorithmic_bytes": int(lens.sum()) * 4 + 2 * n_cols * 2 + len(raw) * 48,
                      "ops_per_cell": 3, "device": info},
         "checks": {"balanced_accuracy": round(acc, 4), "threshold": round(thr, 3)},
@@ -408,6 +524,8 @@ def main():
                                        "decision_flips": flips}}
         fp.close()
     out["forms"] = forms
+    out["reference_kernel"] = reference_kernel_leg(ctx, w.reads, w.ref_levels, stream, peak_ops)
+    print("[bench] reference launch done", file=sys.stderr, flush=True)
 
     if not a.no_cpu:
         norm = d_norm.to_host()
@@ -416,12 +534,20 @@ def main():
         out["cpu_baseline"] = {"value": round(rate / 1e9, 2), "unit": "GCUPS", "cores": used,
                                "kind": "port",
                                "sample": f"{n_sample} of the {len(qs)} normalised reads, FP32, "
-                                         f"{tc:.1f} s on {used} threads"}
+                                         f"{tc:.1f} s on {used} threads",
+                               **host_description()}
+    plan.close()
+    for arr in (d_raw, d_off, d_len, d_norm, d_out):
+        arr.free()
+    if not a.no_c4:
+        print("[bench] uneven 5 Mb leg", file=sys.stderr, flush=True)
+        c4 = uneven_leg(a, 1, 0, ctx)
+        out["uneven_5mb"] = {k: c4[k] for k in ("value", "unit", "ms_per_step", "kernel_ms", "reads_classified_per_s",
+                                                "config", "roofline", "autotune", "checks")}
+        if "cpu_baseline" in c4:
+            out["uneven_5mb"]["cpu_baseline"] = c4["cpu_baseline"]
     if not a.no_ru:
-        plan.close()
-        for arr in (d_raw, d_off, d_len, d_norm, d_out):
-            arr.free()
-        out["read_until"] = read_until_leg(ctx, a.ru_seconds)
+        out["read_until"] = read_until_leg(ctx, a.ru_seconds, burst_seconds=a.burst_seconds)
     print(json.dumps(out))
 
 

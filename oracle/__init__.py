@@ -258,14 +258,55 @@ def batch_f16(queries_h, strands_h, fused=False, n_threads=0):
     return sc.reshape(n, k), am.reshape(n, k)
 
 
-def cpu_baseline_f32(queries, strands, n_threads=0):
+_BASELINE = None
+_V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
+
+
+def cpu_flags():
+    """(CPU model name, set of flags) of this host, from /proc/cpuinfo."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return model, flags
+
+
+def baseline_lib():
+    """(library, ISA name) for the CPU baseline: the x86-64-v4 (AVX-512)
+    build when this host has those instructions and it is built, else the
+    portable AVX2 build (the checker's own library)."""
+    global _BASELINE
+    if _BASELINE is None:
+        _, flags = cpu_flags()
+        path = os.path.join(_HERE, "liboracle_v4.so")
+        if all(f in flags for f in _V4_FLAGS) and os.path.exists(path):
+            L = C.CDLL(path)
+            L.orc_cpu_baseline_f32.restype = C.c_int
+            L.orc_cpu_baseline_f32.argtypes = lib().orc_cpu_baseline_f32.argtypes
+            _BASELINE = (L, "x86-64-v4 (AVX-512)")
+        else:
+            _BASELINE = (lib(), "x86-64-v3 (AVX2, FMA, F16C)")
+    return _BASELINE
+
+
+def cpu_baseline_f32(queries, strands, n_threads=0, portable=False):
+    """FP32 CPU port of the recurrence (OpenMP over reads, SIMD over columns,
+    column strips for long references) -> (scores, argmins, threads used).
+    Runs the build for this host's best ISA unless portable=True."""
     strands = np.ascontiguousarray(np.atleast_2d(strands), np.float32)
     flat, offs, lens = _pack(queries, np.float32)
     n, k = len(queries), strands.shape[0]
     sc = np.empty(n * k, np.float32)
     am = np.empty(n * k, np.int64)
-    used = lib().orc_cpu_baseline_f32(flat, offs, lens, n, strands.ravel(), strands.shape[1],
-                                      k, sc, am, n_threads)
+    L = lib() if portable else baseline_lib()[0]
+    used = L.orc_cpu_baseline_f32(flat, offs, lens, n, strands.ravel(), strands.shape[1],
+                                  k, sc, am, n_threads)
     return sc.reshape(n, k), am.reshape(n, k), used
 
 

@@ -317,29 +317,44 @@ void orc_batch_f16(const uint16_t* q, const int64_t* offsets,
 /* ------------------------------------------------- FP32 CPU baseline ----
  * Same arithmetic as orc_sdtw_f32 (built with -ffp-contract=off so t*t + m
  * is never fused), written so the column loop vectorises: the row update
- * has no intra-row dependency under NO_REF_DEL.                            */
+ * has no intra-row dependency under NO_REF_DEL.  Long references are swept
+ * in vertical strips of BASELINE_STRIP columns so the two rolling rows stay
+ * in cache: a strip runs every query row, carrying its last column's value
+ * per row (bnd[r] = D[r][c0 + W - 1]) into the next strip as that strip's
+ * column c0 - 1 (the diagonal of its first column).  The cells and their
+ * operations are the same as one pass over whole rows; strips are scanned
+ * left to right, so the first minimum (lowest column) is kept.            */
+#define BASELINE_STRIP 8192
 static void baseline_one(const float* q, int64_t Q, const float* s, int64_t N,
-                         float* a, float* b, float* score, int64_t* argmin) {
-  a[0] = b[0] = INFINITY;
-  for (int64_t c = 0; c < N; c++) a[c + 1] = 0.0f;
-  for (int64_t r = 0; r < Q; r++) {
-    const float qr = q[r];
-    const float* __restrict__ ap = a;
-    float* __restrict__ bp = b + 1;
-#pragma omp simd
-    for (int64_t c = 0; c < N; c++) {
-      float t = s[c] - qr;
-      float t2 = t * t;
-      float top = ap[c + 1], dg = ap[c];
-      float m = top < dg ? top : dg;
-      bp[c] = t2 + m;
-    }
-    float* tmp = a; a = b; b = tmp;
-  }
+                         float* a, float* b, float* bnd, float* score, int64_t* argmin) {
+  const int64_t W = N > 2 * BASELINE_STRIP ? BASELINE_STRIP : N;
   float best = INFINITY;
   int64_t bi = -1;
-  for (int64_t c = 0; c < N; c++)
-    if (a[c + 1] < best) { best = a[c + 1]; bi = c; }
+  for (int64_t c0 = 0; c0 < N; c0 += W) {
+    const int64_t w = (N - c0 < W) ? N - c0 : W;
+    const float* sp = s + c0;
+    /* row -1: 0 (free start); column c0 - 1 is +inf for the first strip */
+    a[0] = c0 == 0 ? INFINITY : 0.0f;
+    for (int64_t c = 0; c < w; c++) a[c + 1] = 0.0f;
+    for (int64_t r = 0; r < Q; r++) {
+      const float qr = q[r];
+      const float* __restrict__ ap = a;
+      float* __restrict__ bp = b + 1;
+#pragma omp simd
+      for (int64_t c = 0; c < w; c++) {
+        float t = sp[c] - qr;
+        float t2 = t * t;
+        float top = ap[c + 1], dg = ap[c];
+        float m = top < dg ? top : dg;
+        bp[c] = t2 + m;
+      }
+      b[0] = c0 == 0 ? INFINITY : bnd[r]; /* D[r][c0 - 1]: next row's first diagonal */
+      bnd[r] = b[w];                      /* D[r][c0 + w - 1] for the next strip */
+      float* tmp = a; a = b; b = tmp;
+    }
+    for (int64_t c = 0; c < w; c++)
+      if (a[c + 1] < best) { best = a[c + 1]; bi = c0 + c; }
+  }
   *score = best;
   *argmin = bi;
 }
@@ -349,18 +364,23 @@ int orc_cpu_baseline_f32(const float* q, const int64_t* offsets,
                          int64_t N, int n_strands, float* score,
                          int64_t* argmin, int n_threads) {
   int used = set_threads(n_threads);
+  int64_t qmax = 1;
+  for (int64_t r = 0; r < n_reads; r++) qmax = lens[r] > qmax ? lens[r] : qmax;
 #pragma omp parallel
   {
-    float* a = (float*)aligned_alloc(64, sizeof(float) * (size_t)((N + 1 + 15) / 16 * 16));
-    float* b = (float*)aligned_alloc(64, sizeof(float) * (size_t)((N + 1 + 15) / 16 * 16));
+    const int64_t W = N > 2 * BASELINE_STRIP ? BASELINE_STRIP : N;
+    float* a = (float*)aligned_alloc(64, sizeof(float) * (size_t)((W + 1 + 15) / 16 * 16));
+    float* b = (float*)aligned_alloc(64, sizeof(float) * (size_t)((W + 1 + 15) / 16 * 16));
+    float* bnd = (float*)aligned_alloc(64, sizeof(float) * (size_t)((qmax + 15) / 16 * 16));
 #pragma omp for schedule(dynamic, 1)
     for (int64_t i = 0; i < n_reads * n_strands; i++) {
       int64_t r = i / n_strands, k = i % n_strands;
-      baseline_one(q + offsets[r], lens[r], s + k * N, N, a, b, &score[i],
+      baseline_one(q + offsets[r], lens[r], s + k * N, N, a, b, bnd, &score[i],
                    &argmin[i]);
     }
     free(a);
     free(b);
+    free(bnd);
   }
   return used;
 }

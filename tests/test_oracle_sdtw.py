@@ -189,6 +189,21 @@ def test_f16_fused_close_to_fp64():
     assert abs(fa - ex) / ex < 0.05 and abs(fb - ex) / ex < 0.05
 
 
+def test_baseline_strips_and_isa_builds_agree():
+    """The CPU baseline's column strips (a 50,000-column reference is swept
+    in strips of 8192) and its x86-64-v4 build give the oracle's scores and
+    argmins bit for bit."""
+    rng = np.random.default_rng(18)
+    strands = rng.standard_normal((2, 50_000)).astype(np.float32)
+    qs = [rng.standard_normal(n).astype(np.float32) for n in (1, 40, 301)]
+    qs.append(strands[1, 8180:8180 + 30].copy())  # an exact match across a strip edge
+    sc, am = O.batch_f32(qs, strands)
+    for portable in (True, False):
+        sb, ab, _ = O.cpu_baseline_f32(qs, strands, portable=portable)
+        assert np.array_equal(sc, sb) and np.array_equal(am, ab), portable
+    assert sc[3, 1] == 0 and am[3, 1] == 8180 + 29
+
+
 def test_batches_and_baseline_agree():
     rng = np.random.default_rng(8)
     strands = rng.standard_normal((2, 700)).astype(np.float32)
