@@ -524,7 +524,7 @@ class RuOpts(C.Structure):
                 ("precision", C.c_int32), ("segment", C.c_int32), ("norm", C.c_int32),
                 ("eps", C.c_double), ("threshold", C.c_float), ("min_ref_chunk", C.c_int32),
                 ("hbm_budget", C.c_int64), ("threshold_per_row", C.c_float),
-                ("n_streams", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("n_streams", C.c_int32), ("renorm_drift", C.c_float), ("reserved", C.c_int32)]
 
 
 RU_CHUNK_DTYPE = np.dtype([("channel", np.int32), ("number", np.int32), ("offset", np.int64)])
@@ -539,13 +539,14 @@ class ReadUntil:
 
     def __init__(self, ctx, max_reads=512, chunk_len=1600, max_chunks=4, precision=F16,
                  segment=0, norm=NORM_MEDMAD, eps=0.0, threshold=0.0, min_ref_chunk=0,
-                 hbm_budget=0, threshold_per_row=0.0, n_streams=0):
+                 hbm_budget=0, threshold_per_row=0.0, n_streams=0, renorm_drift=0.0):
         if isinstance(precision, str):
             precision = PRECISIONS[precision]
         o = RuOpts(max_reads=max_reads, chunk_len=chunk_len, max_chunks=max_chunks,
                    precision=precision, segment=segment, norm=norm, eps=eps,
                    threshold=threshold, min_ref_chunk=min_ref_chunk, hbm_budget=hbm_budget,
-                   threshold_per_row=threshold_per_row, n_streams=n_streams)
+                   threshold_per_row=threshold_per_row, n_streams=n_streams,
+                   renorm_drift=renorm_drift)
         p = C.c_void_p()
         _check(lib().dtwax_ru_create(ctx.handle, C.byref(o), C.byref(p)))
         self.handle = p.value

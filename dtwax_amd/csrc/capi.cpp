@@ -810,6 +810,7 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
   k.lr_stride = L.lr_stride;
   k.lr_ntiles = L.lr_ntiles;
   k.lr_last = L.lr_last;
+  k.ru_renorm = L.ru_renorm;
   k.n_items = L.n_items;
   k.queries = L.queries;
   k.q_off = L.q_off;

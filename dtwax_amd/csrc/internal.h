@@ -78,6 +78,7 @@ struct ScoreLaunch {
   int32_t lr_ntiles = 0, lr_last = 0;
   void* dists = nullptr;
   bool dists_h2 = false;
+  const uint32_t* ru_renorm = nullptr;  // Read Until DTWAX_F16_RENORM: {K, k0} per read
 };
 
 }  // namespace dtwax

@@ -97,6 +97,12 @@ struct KParams {
   int64_t lr_stride;
   int32_t lr_ntiles;
   int32_t lr_last;
+  // Read Until items of DTWAX_F16_RENORM: per read of the batch two half2
+  // words {K, k0}: K = the per-lane drift subtracted at every lane hop of
+  // every band, k0 = what lane 0 also subtracts from a resumed top row (the
+  // stored row's minimum).  The keys then hold the lowered scores; the
+  // engine adds the offsets back (per-slot, in binary64).
+  const uint32_t* ru_renorm;
 };
 
 // The reference's tile geometry (src/include/common.hpp:63-66, 87):

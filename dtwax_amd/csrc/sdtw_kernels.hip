@@ -745,6 +745,15 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
   float off_f = 0.0f, off_r = 0.0f;
   h2 prev_min = hsplat((half_t)0.0f);
   h2 Kb = K;  // per-lane drift of the current band (adapted band to band)
+  // Read Until: the engine's per-read drift, and the stored row's minimum,
+  // which lane 0 subtracts from the resumed top row; one offset for every
+  // reference piece of the chunk (the pieces share the stored row), so no
+  // band-to-band adaptation; the keys hold the lowered score
+  const bool ru_rn = MODE == DTWAX_F16_RENORM && p.ru_renorm != nullptr;
+  if (ru_rn) {
+    Kb = __builtin_bit_cast(h2, p.ru_renorm[2 * (int64_t)read]);
+    if (trow_ext) prev_min = __builtin_bit_cast(h2, p.ru_renorm[2 * (int64_t)read + 1]);
+  }
 
   for (int32_t b = 0; b < nb; b++) {
     F16Band<S, MODE> B;
@@ -785,7 +794,8 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     }
     h2 rowmin = hsplat((half_t)__builtin_inff());
     h2 Kl = Kb;
-    if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0) Kl = Kb + prev_min;
+    if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0 && !ru_rn) Kl = Kb + prev_min;
+    if (ru_rn && b == 0 && lane == 0) Kl = Kb + prev_min;
     if constexpr (use_band2<S, MODE, M3>())
       f16_band2<S, MODE, QK, FIX>(B, io, ring_s, ring_t, lane, best, posf, posr);
     else
@@ -796,7 +806,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
       const h2 k0 = __builtin_bit_cast(h2, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Kl), first));
       off_f += (float)(63 - first) * (float)Kb.x + (float)k0.x;
       off_r += (float)(63 - first) * (float)Kb.y + (float)k0.y;
-      if (!io.final_band) {
+      if (!io.final_band && !ru_rn) {
         h2 m = rowmin;
 #pragma unroll
         for (int sh = 32; sh >= 1; sh >>= 1) {
@@ -814,6 +824,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
   }
 
   // Combine: per strand min value (as float + offset), lowest column on ties.
+  if (ru_rn) off_f = off_r = 0.0f;  // the engine adds the offsets back
   uint64_t kf = (posf >= 0) ? make_key((float)best.x + off_f, posf) : ~0ull;
   uint64_t kr = (posr >= 0) ? make_key((float)best.y + off_r, posr) : ~0ull;
   publish_min(&p.keys[2 * (int64_t)read], wave_min_u64(kf), lane);

@@ -24,6 +24,20 @@
 // Device memory per slot: 2 rows (ping-pong: chunk k reads row cur and
 // writes row 1-cur, since the pieces of one chunk run concurrently and read
 // the stored row across their overlaps) x (n_cols + pads) x 4 bytes.
+//
+// DTWAX_F16_RENORM: every lane hop subtracts a per-row drift K (binary16,
+// per strand) and lane 0 of a resumed chunk also subtracts k0, the stored
+// row's exact minimum, so each chunk's values start near 0 and binary16
+// keeps its resolution over any number of chunks.  A per-row constant moves
+// every path's cost by the same amount, so argmins are unchanged; the
+// kernel reports the lowered minimum and the engine adds the subtracted
+// amounts back per slot in binary64 (64 K per band + k0 per chunk).  K
+// starts at renorm_drift x segment per lane (0.01 per row by default; a
+// larger start overshoots for targets, whose scores rise slowly, and their
+// relative error grew) and is adapted after every chunk by the
+// lowered minimum's remaining rise, K += m / 64 (the batch form's band-to-
+// band rule); within a chunk it is fixed, so every reference piece of the
+// chunk lowers the stored row by the same amount.
 
 #include <hip/hip_runtime.h>
 
@@ -78,6 +92,8 @@ struct Lane {
   unsigned long long* d_keys = nullptr;
   DeviceResult* d_res = nullptr;
   void* d_scratch = nullptr;
+  uint32_t* d_rn = nullptr;  // DTWAX_F16_RENORM: {K, k0} half2 words per read
+  uint32_t* h_rn = nullptr;
   float* h_raw = nullptr;
   int32_t* h_slot = nullptr;
   int32_t* h_first = nullptr;
@@ -124,6 +140,10 @@ struct dtwax_ru {
   std::unordered_map<int32_t, ChanRec> chans;
   std::vector<int32_t> free_slots;
   std::vector<int32_t> slot_cur;
+  // DTWAX_F16_RENORM per slot: drift K and stored-row minimum k0 (binary16
+  // bits per strand), and the amount subtracted so far (binary64 per strand)
+  std::vector<uint16_t> rn_k, rn_k0;
+  std::vector<double> rn_acc;
   int64_t batches = 0, chunks_scored = 0, cells = 0;
   double gpu_ms = 0.0;
   // serving loop
@@ -143,9 +163,10 @@ namespace {
 void lane_free(Lane& l) {
   if (l.st) (void)hipStreamSynchronize(l.st);
   for (void* p : {(void*)l.d_raw, (void*)l.d_norm, (void*)l.d_slot, (void*)l.d_first, (void*)l.d_items,
-                  (void*)l.d_counter, (void*)l.d_keys, (void*)l.d_res, l.d_scratch})
+                  (void*)l.d_counter, (void*)l.d_keys, (void*)l.d_res, l.d_scratch, (void*)l.d_rn})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)l.h_raw, (void*)l.h_slot, (void*)l.h_first, (void*)l.h_items, (void*)l.h_res})
+  for (void* p : {(void*)l.h_raw, (void*)l.h_slot, (void*)l.h_first, (void*)l.h_items, (void*)l.h_res,
+                  (void*)l.h_rn})
     if (p) (void)hipHostFree(p);
   for (hipEvent_t e : {l.ev0, l.ev1, l.done})
     if (e) (void)hipEventDestroy(e);
@@ -171,6 +192,9 @@ bool lane_alloc(dtwax_ru* r, Lane& l) {
        hipHostMalloc((void**)&l.h_first, slots * sizeof(int32_t), 0) == hipSuccess &&
        hipHostMalloc((void**)&l.h_items, (size_t)r->items_cap * sizeof(WorkItem), 0) == hipSuccess &&
        hipHostMalloc((void**)&l.h_res, slots * sizeof(DeviceResult), 0) == hipSuccess;
+  if (ok && r->o.precision == DTWAX_F16_RENORM)
+    ok = hipMalloc(&l.d_rn, slots * 2 * sizeof(uint32_t)) == hipSuccess &&
+         hipHostMalloc((void**)&l.h_rn, slots * 2 * sizeof(uint32_t), 0) == hipSuccess;
   if (ok && r->multiband) {
     const int64_t n = r->grid_slots * r->scratch_stride * r->epc;
     ok = hipMalloc(&l.d_scratch, (size_t)n * 4) == hipSuccess &&
@@ -210,6 +234,9 @@ void set_decision(dtwax_ru_decision& d, int32_t channel, const ChanRec& c, int32
   d.argmin[1] = c.argmin[1];
   d.latency_us = 0.0;
 }
+
+uint16_t f16_bits(float v) { return __builtin_bit_cast(uint16_t, (_Float16)v); }
+float f16_value(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 
 void release_slot(dtwax_ru* r, ChanRec& c) {
   if (c.slot >= 0) r->free_slots.push_back(c.slot);
@@ -269,6 +296,17 @@ int launch(dtwax_ru* r, Lane& l, const std::vector<int32_t>& channels, const std
     l.h_slot[j] = rec.slot;
     const bool first = rec.chunks == 0;
     l.h_first[j] = first ? 1 : 0;
+    if (r->o.precision == DTWAX_F16_RENORM) {
+      const size_t s2 = 2 * (size_t)rec.slot;
+      if (first) {  // a new read in this slot
+        const uint16_t k = f16_bits(r->o.renorm_drift * (float)r->S);
+        r->rn_k[s2] = r->rn_k[s2 + 1] = k;
+        r->rn_k0[s2] = r->rn_k0[s2 + 1] = 0;
+        r->rn_acc[s2] = r->rn_acc[s2 + 1] = 0.0;
+      }
+      l.h_rn[2 * j] = (uint32_t)r->rn_k[s2] | ((uint32_t)r->rn_k[s2 + 1] << 16);
+      l.h_rn[2 * j + 1] = (uint32_t)r->rn_k0[s2] | ((uint32_t)r->rn_k0[s2 + 1] << 16);
+    }
     const bool store = rec.chunks + 1 < r->o.max_chunks;
     for (int64_t c0 = 0; c0 < N; c0 += piece) {
       WorkItem w{};
@@ -287,6 +325,8 @@ int launch(dtwax_ru* r, Lane& l, const std::vector<int32_t>& channels, const std
   DTWAX_HIP_TRY(hipMemcpyAsync(l.d_slot, l.h_slot, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, l.st));
   DTWAX_HIP_TRY(hipMemcpyAsync(l.d_first, l.h_first, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, l.st));
   DTWAX_HIP_TRY(hipMemcpyAsync(l.d_items, l.h_items, (size_t)ni * sizeof(WorkItem), hipMemcpyHostToDevice, l.st));
+  if (l.d_rn)
+    DTWAX_HIP_TRY(hipMemcpyAsync(l.d_rn, l.h_rn, (size_t)m * 2 * sizeof(uint32_t), hipMemcpyHostToDevice, l.st));
   DTWAX_HIP_TRY(launch_ru_normalize(l.d_raw, (int32_t)m, L, l.d_slot, l.d_first, r->o.norm, r->o.eps,
                                     r->d_stats, l.d_norm, l.st));
   ScoreLaunch sl;
@@ -306,6 +346,8 @@ int launch(dtwax_ru* r, Lane& l, const std::vector<int32_t>& channels, const std
   sl.scratch_stride = r->scratch_stride;
   sl.state = (char*)r->d_state + (int64_t)COL_PADL * r->epc * 4;  // column 0 of slot 0, row 0
   sl.row_stride = r->row_stride;
+  sl.ru_renorm = l.d_rn;
+  sl.renorm_k = r->o.renorm_drift * (float)r->S;
   sl.out = l.d_res;
   sl.ev0 = l.ev0;
   sl.ev1 = l.ev1;
@@ -338,6 +380,25 @@ int retire(dtwax_ru* r, Lane& l, dtwax_ru_decision* out) {
     for (int k = 0; k < 2; k++) {
       rec.score[k] = (k < r->nstr) ? res.score[k] : INFINITY;
       rec.argmin[k] = (k < r->nstr) ? res.argmin[k] : -1;
+    }
+    if (r->o.precision == DTWAX_F16_RENORM) {
+      // add back what this chunk subtracted (64 K per band, k0 at lane 0 of
+      // a resumed chunk) and the earlier chunks' amounts; then the next
+      // chunk's k0 (this row's lowered minimum) and adapted drift
+      const size_t s2 = 2 * (size_t)rec.slot;
+      const int64_t bands = (int64_t)L / (64 * r->S);
+      for (int k = 0; k < r->nstr; k++) {
+        const double K = f16_value(r->rn_k[s2 + k]);
+        r->rn_acc[s2 + k] += (double)bands * 64.0 * K + (double)f16_value(r->rn_k0[s2 + k]);
+        const float lowered = res.score[k];
+        rec.score[k] = std::isfinite(lowered) ? (float)((double)lowered + r->rn_acc[s2 + k]) : lowered;
+        if (std::isfinite(lowered)) {
+          r->rn_k0[s2 + k] = f16_bits(lowered);  // a binary16 value already
+          r->rn_k[s2 + k] = f16_bits((float)K + lowered / 64.0f);
+        } else {
+          r->rn_k0[s2 + k] = 0;
+        }
+      }
     }
     const float best = std::min(rec.score[0], rec.score[1]);
     const float limit = r->o.threshold_per_row > 0.0f
@@ -480,8 +541,8 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   if (o.max_reads < 1 || o.chunk_len < 64 || o.max_chunks < 1 || o.n_streams < 1 || o.n_streams > 8)
     return set_error(DTWAX_EINVAL, "max_reads, chunk_len (>= 64), max_chunks and n_streams (1..8) "
                                    "must be positive");
-  if (o.precision != DTWAX_F16 && o.precision != DTWAX_F32)
-    return set_error(DTWAX_EINVAL, "Read Until serving supports DTWAX_F16 and DTWAX_F32");
+  if (o.precision != DTWAX_F16 && o.precision != DTWAX_F32 && o.precision != DTWAX_F16_RENORM)
+    return set_error(DTWAX_EINVAL, "Read Until serving supports DTWAX_F16, DTWAX_F16_RENORM and DTWAX_F32");
   if (o.norm < DTWAX_NORM_NONE || o.norm > DTWAX_NORM_MEDMAD) return set_error(DTWAX_EINVAL, "bad norm");
   if (o.chunk_len % 64) return set_error(DTWAX_EINVAL, "chunk_len must be a multiple of 64");
   const int rows_per_lane = o.chunk_len / 64;
@@ -496,6 +557,7 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   if (o.min_ref_chunk < 0 || o.hbm_budget < 0) return set_error(DTWAX_EINVAL, "negative size");
   if (o.min_ref_chunk == 0) o.min_ref_chunk = 4 * o.chunk_len;
   if (!(o.threshold > 0.0f)) o.threshold = INFINITY;  // no early non-target decision
+  if (!(o.renorm_drift > 0.0f)) o.renorm_drift = 0.01f;
 
   auto* r = new dtwax_ru;
   r->ctx = c;
@@ -553,6 +615,11 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   r->free_slots.reserve((size_t)slots);
   for (int64_t s = slots - 1; s >= 0; s--) r->free_slots.push_back((int32_t)s);
   r->slot_cur.assign((size_t)slots, 0);
+  if (o.precision == DTWAX_F16_RENORM) {
+    r->rn_k.assign(2 * (size_t)slots, 0);
+    r->rn_k0.assign(2 * (size_t)slots, 0);
+    r->rn_acc.assign(2 * (size_t)slots, 0.0);
+  }
   *out = r;
   return DTWAX_OK;
 }

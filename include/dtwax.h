@@ -386,7 +386,9 @@ typedef struct {
   int32_t max_reads;     /* state slots = reads undecided at once (0: 512) */
   int32_t chunk_len;     /* samples per chunk, a multiple of 64 (0: 1600) */
   int32_t max_chunks;    /* chunks before a TARGET decision (0: 4) */
-  int32_t precision;     /* DTWAX_F16 or DTWAX_F32 */
+  int32_t precision;     /* DTWAX_F16, DTWAX_F16_RENORM (each chunk subtracts a per-row drift
+                            at its lane hops and the stored row's minimum at lane 0; the
+                            engine adds the amounts back in binary64) or DTWAX_F32 */
   int32_t segment;       /* rows per lane, dividing chunk_len / 64 (0: largest of 32, 28, 25, 20, 16, 12, 8) */
   int32_t norm;          /* dtwax_norm_mode, statistics of the read's first chunk */
   double eps;            /* DTWAX_NORM_MEANSTD epsilon */
@@ -397,7 +399,9 @@ typedef struct {
                               threshold_per_row * k * chunk_len (replaces threshold) */
   int32_t n_streams;     /* serving loop: batches in flight at once, each on its own HIP
                             stream (1..8; 0: 2) */
-  int32_t reserved[2];
+  float renorm_drift;    /* DTWAX_F16_RENORM: initial per-row drift subtracted at lane hops,
+                            adapted chunk to chunk (<= 0: 0.01) */
+  int32_t reserved;
 } dtwax_ru_opts;
 
 typedef struct {
