@@ -179,7 +179,9 @@ int dtwax_autotune(dtwax_ctx* c, const float* d_queries, const int64_t* d_offset
   }
 
   // 0: per-read widths (the plan's own choice)
-  std::vector<int32_t> segs = (o.precision == DTWAX_F16_RENORM) ? std::vector<int32_t>{8, 16}
+  // the renormalised form: its own per-read choice (bands short enough for
+  // its precision) or the shortest bands for every read
+  std::vector<int32_t> segs = (o.precision == DTWAX_F16_RENORM) ? std::vector<int32_t>{0, 8}
                                                                  : std::vector<int32_t>{0, 8, 12, 16, 20, 25, 28, 32};
   // packing = 1 packs at the widths that have a packed kernel only (automatic, 16..32)
   if (o.packing == 1) segs.erase(std::remove_if(segs.begin(), segs.end(), [](int32_t S) { return S && S < 16; }),

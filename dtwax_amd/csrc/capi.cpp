@@ -425,15 +425,22 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   const float drift = o.renorm_drift > 0 ? o.renorm_drift : 0.01f;
   const int64_t N = col_limit;
 
-  // Segment width per read.  Fixed when given; the renormalised form
-  // subtracts each band's exact row minimum, so it defaults to the shortest
-  // bands (64 x 8 rows); binary32 takes one width for the batch; the
-  // binary16 forms pick, per read, the width with the least padded work.
+  // Segment width per read.  Fixed when given; binary32 takes one width for
+  // the batch; the binary16 forms pick, per read, the width with the least
+  // padded work.  The renormalised form subtracts each band's exact row
+  // minimum from the next band, so its first band must be short enough that
+  // binary16 has not drifted yet: reads of up to 1536 rows take bands of
+  // 64 x 8 rows (two or more bands); longer ones run the S = 16 kernel's
+  // split schedule (their first 513..1536 rows in bands of 64 x 8 rows, the
+  // rest in bands of 64 x 16 rows on the two-column sweep).  Against FP64
+  // (max relative error of min(fwd, rev), 48 reads of the 48.5 kb workload):
+  // 2000 rows at S = 8 0.36%, split 0.46%, S = 16 throughout 1.25%; 3000
+  // rows 0.52%, 0.46%, 0.86%.
   std::vector<int> seg_of((size_t)n_reads, 0);
   if (o.segment) {
     std::fill(seg_of.begin(), seg_of.end(), o.segment);
   } else if (o.precision == DTWAX_F16_RENORM) {
-    std::fill(seg_of.begin(), seg_of.end(), 8);
+    for (int32_t r = 0; r < n_reads; r++) seg_of[(size_t)r] = lens[r] <= 1536 ? 8 : 16;
   } else if (o.precision == DTWAX_F32) {
     std::fill(seg_of.begin(), seg_of.end(), auto_segment(lens));
   } else {
@@ -540,7 +547,13 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     for (int32_t r = 0; r < n_reads; r++) {
       if (seg_of[(size_t)r] != S || packed_read[(size_t)r]) continue;
       const int64_t q = lens[r];
-      const int64_t nb = (q + BR - 1) / BR;
+      // rows computed, virtual ones included (the split schedule's first
+      // part pads to 512 rows)
+      int64_t rows = (q + BR - 1) / BR * BR;
+      if (o.precision == DTWAX_F16_RENORM && !o.segment && S == 16 && q > 1536) {
+        const int64_t rest = 1024 * ((q - 513) / 1024);
+        rows = rest + (q - rest + 511) / 512 * 512;
+      }
       for (int64_t c0 = 0; c0 < N; c0 += chunk) {
         const int64_t c1 = std::min<int64_t>(N, c0 + chunk);
         const int64_t x = std::max<int64_t>(0, c0 - (q - 1));
@@ -551,10 +564,10 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
         it.c0 = (int32_t)c0;
         it.c1 = (int32_t)c1;
         const int64_t steps = ((c1 - x + 63) + 63) / 64 * 64;
-        const int64_t cost = nb * steps;
+        const int64_t cost = rows * steps;
         items.push_back({cost, it});
         p->cells += q * (c1 - c0) * strands_counted;
-        p->computed += nb * BR * steps * (p->precision == DTWAX_F32 ? p->nstr : 2);
+        p->computed += rows * steps * (p->precision == DTWAX_F32 ? p->nstr : 2);
         max_w = std::max(max_w, c1 - x);
       }
     }
@@ -570,6 +583,10 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
     g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(slots_S, (int64_t)items.size()));
     if (o.waves > 0) g.grid = std::min(g.grid, o.waves);
     g.renorm_k = drift * (float)S;
+    if (o.precision == DTWAX_F16_RENORM && !o.segment && S == 16) {
+      g.renorm_split = true;  // starts at 64 x 8 rows per band
+      g.renorm_k = drift * 8.0f;
+    }
     g.scratch_wg0 = scratch_rows;  // concurrent groups: disjoint band scratch
     scratch_rows += g.grid;
     max_grid = std::max(max_grid, g.grid);
@@ -838,6 +855,7 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
       k.n_items = G.n_items;
       k.counter = L.counter + g;
       k.renorm_k = G.renorm_k;
+      k.renorm_split = G.renorm_split ? 1 : 0;
       k.scratch = L.scratch ? (void*)((float*)L.scratch + G.scratch_wg0 * L.scratch_stride * epc) : nullptr;
       k.scratch_rows = G.rows;
       k.pack_lanes = L.pack_lanes;

@@ -27,6 +27,7 @@ struct LaunchGroup {
   int32_t n_items = 0;
   int grid = 1;
   float renorm_k = 0.f;
+  bool renorm_split = false;  // KParams::renorm_split
   int64_t scratch_wg0 = 0;  // first band-scratch row of this group
   bool pack = false;        // packed items (sdtw_f16_pack_kernel)
   int32_t rows = 1;         // band scratch rows per workgroup

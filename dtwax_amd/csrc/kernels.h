@@ -83,6 +83,10 @@ struct KParams {
                              // starts (2*s + k) * row_stride columns further
   int64_t row_stride;        // columns per state row (includes the pads)
   float renorm_k;            // F16_RENORM per-lane drift constant (S rows' worth)
+  // F16_RENORM at S = 16 (not Read Until): nonzero runs the split schedule,
+  // the first Q - renorm_split_rows(Q) rows in bands of 64 x 8 rows from
+  // drift renorm_k, the rest in bands of 64 x 16 rows at twice the drift
+  int32_t renorm_split;
   const PackLane* pack_lanes;  // 64 per pack (packed launches)
   const PackRead* pack_reads;
   int32_t scratch_rows;        // band scratch rows per workgroup (packed launches; else 1)

@@ -34,6 +34,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace dtwax {
@@ -382,10 +384,14 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
 // runs at S = 32 as before (17.9) and 4.5% faster at S = 16.  At S = 8 the
 // doubled per-step overhead (four lane moves per 16 cells, a 127-step ramp
 // per band) outweighs it (15.9 -> 14.4), so S = 8 keeps the one-column
-// sweep, as do the renormalised form and the three-move recurrence.
+// sweep, as does the three-move recurrence.  The renormalised form lowers
+// each lane hop's two top inputs by the lane's drift (two packed
+// subtractions per step, where the one-column sweep has one) and takes the
+// band's last-row minimum in the history flush; its cells and their
+// operations are the one-column sweep's, so the results are identical.
 template <int S, int MODE, bool M3>
 constexpr bool use_band2() {
-  return S >= 16 && !M3 && (MODE == DTWAX_F16 || MODE == DTWAX_F16_FUSED);
+  return S >= 16 && !M3 && (MODE == DTWAX_F16 || MODE == DTWAX_F16_FUSED || MODE == DTWAX_F16_RENORM);
 }
 constexpr int RING2 = 512;    // slots; column offset o = c - x lives in slot (o + 128) & 511
 constexpr int MIRROR2 = 136;  // slots [0, MIRROR2) are also kept at RING2 + slot
@@ -468,9 +474,14 @@ constexpr int QK_PERIOD = TILE_W / 2;  // steps per tile (two columns per step)
 // One band over columns [x, c1) (+ ramp), two columns per step; band-final
 // min/argmin per strand over [c0, c1) into best/posf/posr, the band's last
 // row to io.orow.
+// DTWAX_F16_RENORM: K is the lane's drift, subtracted from both top inputs
+// at every lane hop (lane 0: from the band's top row); rowmin receives the
+// band's last-row minimum over [x, c1) (not the final band).
 template <int S, int MODE, bool QK = false, bool FIX = false>
 __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const SweepIO& io, h2* ring_s,
-                                          h2* ring_t, int lane, h2& best, int32_t& posf, int32_t& posr) {
+                                          h2* ring_t, int lane, h2& best, int32_t& posf, int32_t& posr,
+                                          h2 K = (h2){(half_t)0.0f, (half_t)0.0f}, h2* rowmin = nullptr) {
+  constexpr bool RN = MODE == DTWAX_F16_RENORM;
   static_assert(!QK || (S == 32 && MODE == DTWAX_F16 && FIX),
                 "seam deviations: binary16 per op, 64 x 32 rows, whole strands from column 0");
   const h2 INF = hsplat((half_t)__builtin_inff());
@@ -511,7 +522,13 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
     tinA = ring_t[128];
     tinB = ring_t[129];
     if (io.top_diag) diagA = ring_t[127];
+    if constexpr (RN) {
+      tinA = tinA - K;
+      tinB = tinB - K;
+      diagA = diagA - K;
+    }
   }
+  h2 rmin = INF;
   const int base0 = (128 - 2 * lane) & (RING2 - 1);
   h2 sA = ring_s[base0], sB = ring_s[base0 + 1];
 
@@ -570,6 +587,10 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
         diagA = tinB;
         tinA = dpp_shr1(tfA, lastA);
         tinB = dpp_shr1(tfB, Bc[S - 1]);
+        if constexpr (RN) {
+          tinA = tinA - K;
+          tinB = tinB - K;
+        }
         histA = dpp_shl1(lastA, histA);
         histB = dpp_shl1(Bc[S - 1], histB);
         sA = nA;
@@ -587,6 +608,8 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
         if (v.y < best.y) { best.y = v.y; posr = col; }
       }
       if (io.orow && col >= x && col < c1 && (!io.final_band || col >= c0)) io.orow[col] = v;
+      if constexpr (RN)
+        if (!io.final_band && col >= x && col < c1) rmin = hmin2(rmin, v);
       if constexpr (FIX)
         if (io.lrow && io.final_band && col >= c0 && col < c1)
           io.lrow[tiled_index(io.lr_ntiles, io.lr_last, col)] = v;
@@ -604,6 +627,8 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
       if (tmod >= QK_PERIOD) tmod -= QK_PERIOD;
     }
   }
+  if constexpr (RN)
+    if (rowmin) *rowmin = rmin;
 }
 
 // The shared band of a pack: f16_band2 with every read's lanes independent.
@@ -714,6 +739,117 @@ __device__ __forceinline__ void f16_band2_pack(const F16Band<S, MODE>& B, const 
 
 __device__ __forceinline__ half_t clamp_half(float v) { return (half_t)v; }
 
+// The renormalised form's split schedule for reads over 1536 rows: the rows
+// after the first 513..1536 (whose bands are 64 x 8 rows) run in bands of
+// 64 x 16 rows, so the first exact minimum subtraction comes within 512
+// rows and the long bands start from an adapted drift.
+__device__ __forceinline__ int32_t renorm_split_rows(int32_t Q) {
+  return Q <= 1536 ? 0 : 1024 * ((Q - 513) / 1024);
+}
+
+// One band of SB rows per lane, its first lane's first row being read row r0
+// (rows < 0 are virtual: zero cost).
+template <int SB, int MODE, bool M3, bool QK, bool FIX>
+__device__ __forceinline__ void f16_item_band(const float* __restrict__ qp, const uint32_t* __restrict__ qw,
+                                              int32_t r0, const SweepIO& io, h2 Kl, h2* ring_s, h2* ring_t,
+                                              int lane, h2& best, int32_t& posf, int32_t& posr, h2& rowmin) {
+  F16Band<SB, MODE> B;
+#pragma unroll
+  for (int i = 0; i < SB; i++) {
+    const int32_t r = r0 + i;
+    if (r >= 0) {
+      // queries are read once: a non-temporal load keeps them from evicting
+      // the reference, which every work item re-reads, from L2
+      if constexpr (FIX) {
+        B.Qn[i] = -__builtin_bit_cast(h2, __builtin_nontemporal_load(&qw[r]));
+      } else {
+        half_t q = (half_t)__builtin_nontemporal_load(&qp[r]);
+        B.Qn[i] = hsplat(-q);
+      }
+      B.A[i] = hsplat((half_t)1.0f);
+    } else {
+      B.Qn[i] = hsplat((half_t)0.0f);
+      B.A[i] = hsplat((half_t)0.0f);
+    }
+  }
+  if constexpr (use_band2<SB, MODE, M3>())
+    f16_band2<SB, MODE, QK, FIX>(B, io, ring_s, ring_t, lane, best, posf, posr, Kl, &rowmin);
+  else
+    f16_sweep<SB, MODE, M3>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
+}
+
+// The renormalised form at S = 16 with the split schedule (KParams::
+// renorm_split; whole reads, not Read Until): the first
+// Q1 = Q - renorm_split_rows(Q) rows in front-padded bands of 64 x 8 rows on
+// the one-column sweep from drift renorm_k, then bands of 64 x 16 rows on
+// the two-column sweep at twice the adapted drift.  Each band's cells and
+// renormalisation are those of a band of its width in f16_item.
+template <int MODE, bool M3>
+__device__ void f16_item_split(const KParams& p, const WorkItem& it, int lane, h2* ring_s, h2* ring_t,
+                               h2* scratch) {
+  static_assert(MODE == DTWAX_F16_RENORM, "the split schedule is the renormalised form's");
+  constexpr int S1 = 8, S2 = 16;
+  constexpr int BR1 = 64 * S1, BR2 = 64 * S2;
+  const int32_t read = it.read;
+  const int32_t Q = p.q_len[read];
+  const float* __restrict__ qp = p.queries + p.q_off[read];
+  const int32_t Q1 = Q - renorm_split_rows(Q);
+  const int32_t nb1 = (Q1 + BR1 - 1) / BR1;
+  const int32_t nb = nb1 + (Q - Q1) / BR2;
+  const int32_t P = nb1 * BR1 - Q1;  // virtual rows in front of band 0
+
+  h2 best = hsplat((half_t)__builtin_inff());
+  int32_t posf = -1, posr = -1;
+  float off_f = 0.0f, off_r = 0.0f;  // the subtracted amounts, added back in binary32
+  h2 prev_min = hsplat((half_t)0.0f);
+  h2 Kb = hsplat((half_t)p.renorm_k);  // per-lane drift (adapted band to band)
+
+  // band b of SB rows per lane whose lane 0 starts at read row r0
+  auto band = [&](auto width, int32_t b, int32_t r0) {
+    constexpr int SB = decltype(width)::value;
+    SweepIO io;
+    io.ref = (const h2*)p.ref;
+    h2* scr = scratch - it.x;  // indexed by absolute column
+    io.trow = (b == 0) ? nullptr : scr;
+    io.final_band = (b == nb - 1);
+    io.orow = io.final_band ? nullptr : scr;
+    io.x = it.x;
+    io.c0 = it.c0;
+    io.c1 = it.c1;
+    io.top_diag = false;
+    io.lrow = nullptr;
+    io.lr_ntiles = io.lr_last = 0;
+    h2 rowmin = hsplat((half_t)__builtin_inff());
+    h2 Kl = Kb;
+    if (b > 0 && lane == 0) Kl = Kb + prev_min;
+    f16_item_band<SB, MODE, M3, false, false>(qp, nullptr, r0 + lane * SB, io, Kl, ring_s, ring_t, lane, best,
+                                             posf, posr, rowmin);
+    const h2 k0 = __builtin_bit_cast(h2, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Kl), 0));
+    off_f += 63.0f * (float)Kb.x + (float)k0.x;
+    off_r += 63.0f * (float)Kb.y + (float)k0.y;
+    if (!io.final_band) {
+      h2 m = rowmin;
+#pragma unroll
+      for (int sh = 32; sh >= 1; sh >>= 1) {
+        h2 o = __builtin_bit_cast(h2, __shfl_xor(__builtin_bit_cast(int, m), sh, 64));
+        m = hmin2(m, o);
+      }
+      prev_min = m;
+      Kb = Kb + m * hsplat((half_t)(1.0f / 64.0f));
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __threadfence_block();
+  };
+  for (int32_t b = 0; b < nb1; b++) band(std::integral_constant<int, S1>{}, b, b * BR1 - P);
+  Kb = Kb + Kb;  // twice the rows per lane
+  for (int32_t b = nb1; b < nb; b++) band(std::integral_constant<int, S2>{}, b, Q1 + (b - nb1) * BR2);
+
+  uint64_t kf = (posf >= 0) ? make_key((float)best.x + off_f, posf) : ~0ull;
+  uint64_t kr = (posr >= 0) ? make_key((float)best.y + off_r, posr) : ~0ull;
+  publish_min(&p.keys[2 * (int64_t)read], wave_min_u64(kf), lane);
+  publish_min(&p.keys[2 * (int64_t)read + 1], wave_min_u64(kr), lane);
+}
+
 template <int S, int MODE, bool M3, bool QK = false, bool FIX = false>
 __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* ring_s, h2* ring_t,
                          h2* scratch) {
@@ -733,6 +869,13 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     h2* base = (h2*)p.state + (int64_t)it.slot * 2 * p.row_stride;
     if (it.flags & ITEM_RESUME) trow_ext = base + (int64_t)it.cur * p.row_stride;
     if (it.flags & ITEM_STORE) orow_ext = base + (int64_t)(1 - it.cur) * p.row_stride;
+  }
+
+  if constexpr (MODE == DTWAX_F16_RENORM && S == 16 && !QK && !FIX) {
+    if (p.renorm_split && !p.state) {
+      f16_item_split<MODE, M3>(p, it, lane, ring_s, ring_t, scratch);
+      return;
+    }
   }
 
   h2 best = hsplat((half_t)__builtin_inff());
@@ -797,7 +940,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0 && !ru_rn) Kl = Kb + prev_min;
     if (ru_rn && b == 0 && lane == 0) Kl = Kb + prev_min;
     if constexpr (use_band2<S, MODE, M3>())
-      f16_band2<S, MODE, QK, FIX>(B, io, ring_s, ring_t, lane, best, posf, posr);
+      f16_band2<S, MODE, QK, FIX>(B, io, ring_s, ring_t, lane, best, posf, posr, Kl, &rowmin);
     else
       f16_sweep<S, MODE, M3>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
     if (MODE == DTWAX_F16_RENORM) {

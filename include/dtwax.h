@@ -79,8 +79,10 @@ typedef struct {
   int32_t precision;     /* dtwax_precision */
   int32_t segment;       /* query rows per lane (8, 12, 16, 20, 25, 28 or 32) for every read; 0 = automatic:
                             per read for DTWAX_F16 and DTWAX_F16_FUSED (the width with the
-                            least padded work, one persistent launch per width used), 8 for
-                            DTWAX_F16_RENORM, one width per batch for DTWAX_F32 */
+                            least padded work, one persistent launch per width used), per
+                            read for DTWAX_F16_RENORM (8 up to 1536 rows; longer reads their
+                            first 513..1536 rows at 8, the rest at 16: a short first band
+                            keeps its precision), one width per batch for DTWAX_F32 */
   int32_t ref_chunk;     /* reference columns per work item; 0 = automatic (whole strand when
                             it fits); chunks overlap by (query length - 1) columns, exact for
                             the no-deletion recurrence */
@@ -262,7 +264,7 @@ int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
  * 512) spread over the sorted lengths, against the first sample_cols
  * reference columns (<= 0: 2^20), replicated until every candidate has at
  * least 6 work items per resident wave (steady state, as in the full run).
- * It times segment in {0 = per-read widths, 8, 12, 16, 20, 25, 28, 32} ({8, 16} for
+ * It times segment in {0 = per-read widths, 8, 12, 16, 20, 25, 28, 32} ({0, 8} for
  * DTWAX_F16_RENORM) at ref_chunk min(65536, sample_cols); the fastest per
  * counted cell wins.  ref_chunk is then the plan's own rule for the whole
  * workload at that width (whole strands, or the longest chunks whose band

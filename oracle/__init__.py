@@ -75,6 +75,14 @@ def lib():
         L.orc_cpu_baseline_f32.restype = C.c_int
         L.orc_cpu_baseline_f32.argtypes = [_f32p, _i64p, _i64p, C.c_int64, _f32p,
                                            C.c_int64, C.c_int, _f32p, _i64p, C.c_int]
+        L.orc_sdtw_f16_renorm.argtypes = [_u16p, C.c_int64, _u16p, C.c_int64, C.c_int, C.c_float,
+                                          C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+        L.orc_sdtw_f16_renorm_split.argtypes = [_u16p, C.c_int64, _u16p, C.c_int64, C.c_float,
+                                                C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+        L.orc_renorm_split_rows.restype = C.c_int64
+        L.orc_renorm_split_rows.argtypes = [C.c_int64]
+        L.orc_batch_f16_renorm.argtypes = [_u16p, _i64p, _i64p, C.c_int64, _u16p, C.c_int64, C.c_int,
+                                           C.c_int, C.c_float, _f32p, _i64p, C.c_int]
         L.orc_emulate_reference.restype = C.c_int
         L.orc_emulate_reference.argtypes = [C.c_void_p, _f32p, _f32p, C.c_int64, _f32p, C.c_void_p,
                                             C.c_int]
@@ -255,6 +263,41 @@ def batch_f16(queries_h, strands_h, fused=False, n_threads=0):
     am = np.empty(n * k, np.int64)
     lib().orc_batch_f16(flat, offs, lens, n, strands.ravel(), strands.shape[1], k,
                         1 if fused else 0, sc, am, n_threads)
+    return sc.reshape(n, k), am.reshape(n, k)
+
+
+def sdtw_f16_renorm(q_h, s_h, S, drift=0.01):
+    """Renormalised binary16 form (bit patterns in) -> (float score, argmin)."""
+    q = np.ascontiguousarray(q_h, np.uint16)
+    s = np.ascontiguousarray(s_h, np.uint16)
+    sc, am = C.c_float(), C.c_int64()
+    lib().orc_sdtw_f16_renorm(q, q.size, s, s.size, S, drift, C.byref(sc), C.byref(am))
+    return sc.value, am.value
+
+
+def sdtw_f16_renorm_split(q_h, s_h, drift=0.01):
+    """The automatic renormalised schedule for long reads: 512-row bands for
+    the first 513..1536 rows, 1024-row bands for the rest."""
+    q = np.ascontiguousarray(q_h, np.uint16)
+    s = np.ascontiguousarray(s_h, np.uint16)
+    sc, am = C.c_float(), C.c_int64()
+    lib().orc_sdtw_f16_renorm_split(q, q.size, s, s.size, drift, C.byref(sc), C.byref(am))
+    return sc.value, am.value
+
+
+def renorm_split_rows(Q):
+    return int(lib().orc_renorm_split_rows(int(Q)))
+
+
+def batch_f16_renorm(queries_h, strands_h, S, drift=0.01, n_threads=0):
+    """S = 0: the automatic schedule (S = 8 up to 1536 rows, else the split)."""
+    flat, offs, lens = _pack(queries_h, np.uint16)
+    strands_h = np.ascontiguousarray(np.atleast_2d(strands_h), np.uint16)
+    n, k = len(queries_h), strands_h.shape[0]
+    sc = np.empty(n * k, np.float32)
+    am = np.empty(n * k, np.int64)
+    lib().orc_batch_f16_renorm(flat, offs, lens, n, strands_h.ravel(), strands_h.shape[1], k, S,
+                               drift, sc, am, n_threads)
     return sc.reshape(n, k), am.reshape(n, k)
 
 

@@ -13,6 +13,7 @@
  *   three-move recurrence          orc_sdtw3_f64 / _f32 / _f16  (sdtw3.c)
  *   full-matrix brute force        orc_sdtw_bruteforce_f64
  *   literal reference kernel       orc_emulate_reference   (src/include/DTW.cu)
+ *   renormalised binary16 form     orc_sdtw_f16_renorm     (renorm.c; no reference counterpart)
  *   numpy-order statistics         orc_numpy_sum / orc_python_znorm
  *   query normalisation            orc_norm_meanstd / orc_norm_medmad
  *   reference squiggle build       orc_squiggle_reference  (load_reference.hpp)
@@ -139,6 +140,23 @@ typedef struct {
 int orc_emulate_reference(const orc_emu_geom* g, const float* ref,
                           const float* queries, int64_t n_reads, float* out,
                           float* last_row, int n_threads);
+
+/* ---- renormalised binary16 form (renorm.c) ------------------------------ */
+/* DTWAX_F16_RENORM of this build for one read and one strand over whole
+ * strands: bands of 64*S rows, a drift subtracted at every lane boundary
+ * (and the previous band's row minimum at lane 0), the amounts added back
+ * in binary32.  score: float; argmin: lowest column of the minimum. */
+void orc_sdtw_f16_renorm(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, int S,
+                         float drift, float* score, int64_t* argmin);
+/* the automatic schedule for reads over 1536 rows: the first Q - split rows
+ * in bands of 64 x 8 rows, the last split rows in bands of 64 x 16 rows */
+int64_t orc_renorm_split_rows(int64_t Q);
+void orc_sdtw_f16_renorm_split(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, float drift,
+                               float* score, int64_t* argmin);
+/* S = 0: the split schedule */
+void orc_batch_f16_renorm(const uint16_t* q, const int64_t* offsets, const int64_t* lens, int64_t n_reads,
+                          const uint16_t* s, int64_t N, int n_strands, int S, float drift, float* score,
+                          int64_t* argmin, int n_threads);
 
 /* ---- statistics and normalisation -------------------------------------- */
 /* numpy's pairwise summation order for a float64 reduction (what np.sum /

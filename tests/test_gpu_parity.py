@@ -291,6 +291,45 @@ def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     assert np.array_equal(sc, sc0) and np.array_equal(am, am0)
 
 
+@pytest.mark.parametrize("segment", [8, 12, 16, 20, 25, 28, 32])
+def test_f16_renorm_bit_exact(ctx, segment):
+    """DTWAX_F16_RENORM equals the oracle's restatement of the renormalised
+    form (drift at every lane boundary, the previous band's row minimum at
+    lane 0, drift adapted band to band, offsets added back in binary32) bit
+    for bit, scores and argmins, over reads of one to several bands."""
+    rng = np.random.default_rng(600 + segment)
+    ref = rng.standard_normal((2, 6000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    BR = 64 * segment
+    lens = [1, 100, BR - 1, BR, BR + 1, 2 * BR + 17, 3 * BR + 5, 5000]
+    reads = [rng.standard_normal(n).astype(np.float32) for n in lens]
+    n_t = min(3 * BR, 4500)  # a noisy cut of the reference, several bands long
+    reads.append(ref[0, 1000:1000 + n_t] + rng.normal(0, 0.2, n_t).astype(np.float32))
+    sc, am = ctx.score(reads, precision="f16_renorm", segment=segment)
+    sh = np.stack([O.f32_to_f16(ref[k]) for k in range(2)])
+    osc, oam = O.batch_f16_renorm([O.f32_to_f16(r) for r in reads], sh, segment)
+    assert np.array_equal(sc, osc), (sc, osc)
+    assert np.array_equal(am, oam)
+
+
+def test_f16_renorm_automatic_split_bit_exact(ctx):
+    """The automatic renormalised plan: reads of up to 1536 rows in bands of
+    64 x 8 rows, longer ones on the split schedule (their first 513..1536
+    rows in bands of 64 x 8, the rest in bands of 64 x 16 at twice the
+    adapted drift), equal to the oracle's restatement bit for bit."""
+    rng = np.random.default_rng(650)
+    ref = rng.standard_normal((2, 7000)).astype(np.float32)
+    ctx.upload_reference(ref)
+    lens = [300, 1536, 1537, 1600, 2000, 2048, 2561, 3585, 5000]
+    reads = [rng.standard_normal(n).astype(np.float32) for n in lens]
+    reads.append(ref[1, 500:4500] + rng.normal(0, 0.2, 4000).astype(np.float32))
+    sc, am = ctx.score(reads, precision="f16_renorm")
+    sh = np.stack([O.f32_to_f16(ref[k]) for k in range(2)])
+    osc, oam = O.batch_f16_renorm([O.f32_to_f16(r) for r in reads], sh, 0)
+    assert np.array_equal(sc, osc), (sc, osc)
+    assert np.array_equal(am, oam)
+
+
 def test_autotune_with_packing_forced(ctx):
     """packing = 1 leaves the widths without a packed kernel (8, 12) out of
     the candidates instead of failing on them; the choice scores exactly."""
