@@ -254,11 +254,12 @@ static bool valid_segment(int S) {
 static int auto_segment(const std::vector<int32_t>& lens) {
   // binary32: one width per batch, the least padded rows x the measured time
   // per computed cell of that width relative to S = 16 (4096 x 2000-sample
-  // batches, band factor taken out: 1.066 at 8, 1.062 at 12, 1.102 at 20,
-  // 1.084 at 25, 1.076 at 28, 1.109 at 32; from S = 20 up 2 waves fit a SIMD,
-  // at S = 16 three) x 1.5% per extra band
+  // batches, band factor taken out: 1.113 at 8, 1.060 at 12, 1.097 at 20,
+  // 0.996 at 25, 0.984 at 28, 1.006 at 32; from S = 20 up 2 waves fit a SIMD,
+  // at S = 16 three, and from S = 25 up the row costs are looked ahead across
+  // steps) x 1.5% per extra band
   static const int cand[] = {8, 12, 16, 20, 25, 28, 32};
-  static const double f0[] = {1.066, 1.062, 1.0, 1.102, 1.084, 1.076, 1.109};
+  static const double f0[] = {1.113, 1.060, 1.0, 1.097, 0.996, 0.984, 1.006};
   double best = 1e300;
   int bs = 16;
   for (int ci = 0; ci < 7; ci++) {

@@ -1097,21 +1097,38 @@ __global__ __launch_bounds__(64) void sdtw_f16_pack_kernel(KParams p) {
 // itself the scheduler emits one strand's chain as a run of dependent ops).
 // Binary32 VALU ops issue at twice the rate of packed binary16 ones, so the
 // dependent pairs need independent work between them to keep the issue rate.
+// From S = 25 up (two waves per SIMD) the look-ahead also runs across
+// steps: the last LA cells' cost slots compute the next column's (s_next)
+// first LA row costs into cn, which the next step starts from, so no chain
+// op runs without independent work beside it (S = 32: 13.2 -> 14.7 T
+// cells/s on the 48.5 kb x 4096 x 2000 batch).  With three waves per SIMD
+// (S = 16) the step's own first costs, computed at its start, hide the lane
+// move better (14.7 against 14.1), so there the look-ahead stays per step.
+constexpr int F32_LA = 2;  // cells of look-ahead for the row costs
+template <int S>
+constexpr bool f32_xstep() { return S >= 25; }
 template <int S, int NSTR, bool M3>
 __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)[S],
-                                          const float (&s)[NSTR], const float (&tin)[NSTR],
+                                          const float (&s)[NSTR], const float (&s_next)[NSTR],
+                                          float (&cn)[NSTR][F32_LA], const float (&tin)[NSTR],
                                           const float (&dg)[NSTR], const float (&Do)[NSTR][S],
                                           float (&Dn)[NSTR][S]) {
-  constexpr int LA = 2;  // cells of look-ahead for the row costs
+  constexpr int LA = F32_LA;
+  constexpr bool XSTEP = f32_xstep<S>();
+  static_assert(S >= LA, "look-ahead within one column");
   float tp[NSTR], dd[NSTR], c[NSTR][S];
 #pragma unroll
   for (int st = 0; st < NSTR; st++) {
     tp[st] = tin[st];
     dd[st] = dg[st];
 #pragma unroll
-    for (int i = 0; i < LA && i < S; i++) {
-      const float t = __builtin_fmaf(s[st], A[i], Qn[i]);  // == rn(s - q) for a real row
-      c[st][i] = t * t;
+    for (int i = 0; i < LA; i++) {
+      if constexpr (XSTEP) {
+        c[st][i] = cn[st][i];
+      } else {
+        const float t = __builtin_fmaf(s[st], A[i], Qn[i]);  // == rn(s - q) for a real row
+        c[st][i] = t * t;
+      }
     }
   }
   __builtin_amdgcn_sched_barrier(0);
@@ -1125,8 +1142,11 @@ __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)
     }
     float t[NSTR];
 #pragma unroll
-    for (int st = 0; st < NSTR; st++)
+    for (int st = 0; st < NSTR; st++) {
+      // == rn(s - q) for a real row
       if (i + LA < S) t[st] = __builtin_fmaf(s[st], A[i + LA], Qn[i + LA]);
+      else if (XSTEP) t[st] = __builtin_fmaf(s_next[st], A[i + LA - S], Qn[i + LA - S]);
+    }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int st = 0; st < NSTR; st++) {
@@ -1136,8 +1156,10 @@ __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)
       dd[st] = Do[st][i];
     }
 #pragma unroll
-    for (int st = 0; st < NSTR; st++)
+    for (int st = 0; st < NSTR; st++) {
       if (i + LA < S) c[st][i + LA] = t[st] * t[st];
+      else if (XSTEP) cn[st][i + LA - S] = t[st] * t[st];
+    }
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -1148,8 +1170,8 @@ __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)
 template <int S, int NSTR, bool M3>
 __device__ __forceinline__ void f32_step(const float (&A)[S], const float (&Qn)[S], const float* rs,
                                          const float* rt, int u, float (&s_cur)[NSTR],
-                                         float (&tf_cur)[NSTR], float (&tin)[NSTR],
-                                         float (&dg)[NSTR], float (&hist)[NSTR],
+                                         float (&tf_cur)[NSTR], float (&cn)[NSTR][F32_LA],
+                                         float (&tin)[NSTR], float (&dg)[NSTR], float (&hist)[NSTR],
                                          const float (&Do)[NSTR][S], float (&Dn)[NSTR][S]) {
   float s_next[NSTR], tf_next[NSTR];
 #pragma unroll
@@ -1157,7 +1179,7 @@ __device__ __forceinline__ void f32_step(const float (&A)[S], const float (&Qn)[
     s_next[st] = rs[(u + 1) * NSTR + st];
     tf_next[st] = rt[(u + 2) * NSTR + st];
   }
-  f32_cells<S, NSTR, M3>(A, Qn, s_cur, tin, dg, Do, Dn);
+  f32_cells<S, NSTR, M3>(A, Qn, s_cur, s_next, cn, tin, dg, Do, Dn);
 #pragma unroll
   for (int st = 0; st < NSTR; st++) {
     dg[st] = tin[st];
@@ -1239,12 +1261,18 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
       if (lane == 0 && b == 0 && trow_ext) dg[st] = ring_t[63 * NSTR + st];
     }
 
-    // column of step 0 and the top input of step 1 (then read a step ahead)
-    float s_cur[NSTR], tf_cur[NSTR];
+    // column of step 0 and the top input of step 1 (then read a step ahead),
+    // and step 0's first row costs (the look-ahead across steps)
+    float s_cur[NSTR], tf_cur[NSTR], cn[NSTR][F32_LA];
 #pragma unroll
     for (int st = 0; st < NSTR; st++) {
       s_cur[st] = ring_s[((64 - lane) & (RING - 1)) * NSTR + st];
       tf_cur[st] = ring_t[(((64 - lane) & (RING - 1)) + 1) * NSTR + st];
+#pragma unroll
+      for (int i = 0; i < F32_LA; i++) {
+        const float t = f32_xstep<S>() ? __builtin_fmaf(s_cur[st], A[i], Qn[i]) : 0.0f;
+        cn[st][i] = t * t;
+      }
     }
     for (int32_t T = 0; T < nsteps; T += 64) {
       float pf_s[NSTR], pf_t[NSTR];
@@ -1260,8 +1288,8 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
         const float* rt = ring_t + (base + g) * NSTR;
 #pragma unroll
         for (int u = 0; u < 8; u += 2) {
-          f32_step<S, NSTR, M3>(A, Qn, rs, rt, u, s_cur, tf_cur, tin, dg, hist, D0, D1);
-          f32_step<S, NSTR, M3>(A, Qn, rs, rt, u + 1, s_cur, tf_cur, tin, dg, hist, D1, D0);
+          f32_step<S, NSTR, M3>(A, Qn, rs, rt, u, s_cur, tf_cur, cn, tin, dg, hist, D0, D1);
+          f32_step<S, NSTR, M3>(A, Qn, rs, rt, u + 1, s_cur, tf_cur, cn, tin, dg, hist, D1, D0);
         }
       }
       const int32_t col = x + T + lane - 63;
