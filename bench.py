@@ -91,6 +91,12 @@ def measured_traffic():
         return None
 
 
+def valu_peak_ops(ctx):
+    """Packed-VALU element ops/s of the device: N_CU x 64 lanes x 2 halves x clock."""
+    info = ctx.device_info()
+    return info["n_cu"] * 64 * 2 * info["clock_khz"] * 1e3
+
+
 def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
     """Read Until serving on the 5 Mb workload: 512 channels, 1600-sample
     chunks (0.4 s at 4 kHz), decisions within 4 chunks, FP16, median/MAD of
@@ -153,6 +159,9 @@ def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
         "saturated_batch": {"chunks": 512, "wall_ms": round(wall * 1e3, 2),
                             "kernel_ms": round(kms, 2),
                             "kernel_gcups": round(cells / (kms * 1e-3) / 1e9, 1),
+                            # 3 element ops per cell against packed VALU
+                            "roofline_frac": round(3.0 * cells / (kms * 1e-3) / valu_peak_ops(ctx), 4),
+                            "exact_form_ceiling_ms": round(4.0 * cells / valu_peak_ops(ctx) * 1e3, 1),
                             "chunks_per_s": round(512 / wall, 1)},
         "plain_batched_form": plain,
         "serving_vs_plain_reads_per_s": round(m["reads_decided_per_s"] / max(plain["reads_decided_per_s"], 1e-9), 3),
@@ -481,6 +490,9 @@ def main():
                    "segment": pinfo["segment"], "work_items": pinfo["n_items"],
                    "grid": pinfo["grid"], "packed_bands": pinfo["packs"], "packed_reads": pinfo["packed_reads"]},
         "reads_classified_per_s": round(len(raw) / (step_ms * 1e-3), 1),
+        # the reference's own throughput figure (hpc_helpers.hpp:93-100): query
+        # samples per microsecond of the sDTW launch / 10 (~10 samples per base)
+        "reference_mbps": round(float(lens.sum()) / (kms * 1e3) / 10, 1),
         "kernel_ms": round(kms, 3),
         "kernel_gcups": round(cell_rate / 1e9, 1),
         "roofline": {"bound": "valu", "achieved": round(achieved_ops / 1e12, 2),

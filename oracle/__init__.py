@@ -306,17 +306,24 @@ _V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
 
 
 def cpu_flags():
-    """(CPU model name, set of flags) of this host, from /proc/cpuinfo."""
-    model, flags = "unknown", set()
+    """(CPU model name, set of flags) of this host, from /proc/cpuinfo; the
+    name carries the CPU family and model numbers (a virtualised host may
+    report only a brand string)."""
+    model, flags, fam = "unknown", set(), {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name") and model == "unknown":
+                key = line.split(":", 1)[0].strip()
+                if key == "model name" and model == "unknown":
                     model = line.split(":", 1)[1].strip()
-                elif line.startswith("flags") and not flags:
+                elif key in ("cpu family", "model") and key not in fam:
+                    fam[key] = line.split(":", 1)[1].strip()
+                elif key == "flags" and not flags:
                     flags = set(line.split(":", 1)[1].split())
     except OSError:
         pass
+    if fam:
+        model += " (family " + fam.get("cpu family", "?") + ", model " + fam.get("model", "?") + ")"
     return model, flags
 
 
