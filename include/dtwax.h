@@ -54,7 +54,12 @@ typedef enum {
   DTWAX_F16_FUSED = 1,
   /* binary16 per op, with per-row drift subtraction at every lane boundary
    * and exact row-minimum subtraction between bands; the offsets are added
-   * back in binary32. */
+   * back in binary32.  The row minimum is taken over a work item's own
+   * columns, so with the reference in chunks (ref_chunk, or automatic
+   * chunking of small batches and long references) the subtracted constants
+   * and the binary16 rounding depend on the chunking: results are exact for
+   * the chunks as computed, and agree with a whole-strand run to within
+   * rounding, not bit for bit. */
   DTWAX_F16_RENORM = 2,
   /* binary32 per op (the reference's FP32 build, common.hpp:47-57). */
   DTWAX_F32 = 3

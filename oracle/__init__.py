@@ -79,6 +79,8 @@ def lib():
                                           C.POINTER(C.c_float), C.POINTER(C.c_int64)]
         L.orc_sdtw_f16_renorm_split.argtypes = [_u16p, C.c_int64, _u16p, C.c_int64, C.c_float,
                                                 C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+        L.orc_sdtw_f16_renorm_row.argtypes = [_u16p, C.c_int64, _u16p, C.c_int64, C.c_int, C.c_int, C.c_float,
+                                              _f32p, C.POINTER(C.c_float)]
         L.orc_renorm_split_rows.restype = C.c_int64
         L.orc_renorm_split_rows.argtypes = [C.c_int64]
         L.orc_batch_f16_renorm.argtypes = [_u16p, _i64p, _i64p, C.c_int64, _u16p, C.c_int64, C.c_int,
@@ -283,6 +285,17 @@ def sdtw_f16_renorm_split(q_h, s_h, drift=0.01):
     sc, am = C.c_float(), C.c_int64()
     lib().orc_sdtw_f16_renorm_split(q, q.size, s, s.size, drift, C.byref(sc), C.byref(am))
     return sc.value, am.value
+
+
+def sdtw_f16_renorm_row(q_h, s_h, S=8, split=False, drift=0.01):
+    """(lowered last row as float32, binary32 offset) of the renormalised
+    form; score = min(row) + offset at the first minimum."""
+    q = np.ascontiguousarray(q_h, np.uint16)
+    s = np.ascontiguousarray(s_h, np.uint16)
+    row = np.empty(s.size, np.float32)
+    off = C.c_float()
+    lib().orc_sdtw_f16_renorm_row(q, q.size, s, s.size, S, 1 if split else 0, drift, row, C.byref(off))
+    return row, np.float32(off.value)
 
 
 def renorm_split_rows(Q):

@@ -153,6 +153,11 @@ void orc_sdtw_f16_renorm(const uint16_t* q, int64_t Q, const uint16_t* s, int64_
 int64_t orc_renorm_split_rows(int64_t Q);
 void orc_sdtw_f16_renorm_split(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, float drift,
                                float* score, int64_t* argmin);
+/* The last row of either schedule (split: the split schedule from S = 8),
+ * lowered (binary16 values as float), and the binary32 offset the score
+ * adds to its minimum. */
+void orc_sdtw_f16_renorm_row(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, int S, int split,
+                             float drift, float* last_row, float* offset);
 /* S = 0: the split schedule */
 void orc_batch_f16_renorm(const uint16_t* q, const int64_t* offsets, const int64_t* lens, int64_t n_reads,
                           const uint16_t* s, int64_t N, int n_strands, int S, float drift, float* score,

@@ -47,7 +47,8 @@ static uint16_t hmin(uint16_t a, uint16_t b) { return h2f(a) <= h2f(b) ? a : b; 
  * 64*S2 rows, the per-lane drift doubling (S2 = 2S) at the switch.  With
  * rest_rows = 0 every band has 64*S rows. */
 static void renorm_schedule(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, int S, int S2,
-                            int64_t rest_rows, float drift, float* score, int64_t* argmin) {
+                            int64_t rest_rows, float drift, float* score, int64_t* argmin, float* last_row,
+                            float* offset) {
   const uint16_t HINF = 0x7c00;
   const int64_t BR = 64 * (int64_t)S, BR2 = 64 * (int64_t)S2;
   const int64_t Q1 = Q - rest_rows;
@@ -101,13 +102,24 @@ static void renorm_schedule(const uint16_t* q, int64_t Q, const uint16_t* s, int
     if (h2f(a[c + 1]) < best) { best = h2f(a[c + 1]); bi = c; }
   *score = bi >= 0 ? best + off : INFINITY;
   *argmin = bi;
+  if (last_row)
+    for (int64_t c = 0; c < N; c++) last_row[c] = h2f(a[c + 1]);
+  if (offset) *offset = off;
   free(a);
   free(b);
 }
 
 void orc_sdtw_f16_renorm(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, int S,
                          float drift, float* score, int64_t* argmin) {
-  renorm_schedule(q, Q, s, N, S, S, 0, drift, score, argmin);
+  renorm_schedule(q, Q, s, N, S, S, 0, drift, score, argmin, NULL, NULL);
+}
+
+void orc_sdtw_f16_renorm_row(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, int S, int split,
+                             float drift, float* last_row, float* offset) {
+  float sc;
+  int64_t am;
+  renorm_schedule(q, Q, s, N, S, 2 * S, split ? orc_renorm_split_rows(Q) : 0, drift, &sc, &am, last_row,
+                  offset);
 }
 
 int64_t orc_renorm_split_rows(int64_t Q) {
@@ -118,7 +130,7 @@ int64_t orc_renorm_split_rows(int64_t Q) {
 
 void orc_sdtw_f16_renorm_split(const uint16_t* q, int64_t Q, const uint16_t* s, int64_t N, float drift,
                                float* score, int64_t* argmin) {
-  renorm_schedule(q, Q, s, N, 8, 16, orc_renorm_split_rows(Q), drift, score, argmin);
+  renorm_schedule(q, Q, s, N, 8, 16, orc_renorm_split_rows(Q), drift, score, argmin, NULL, NULL);
 }
 
 /* S = 0: the split schedule of orc_sdtw_f16_renorm_split */

@@ -69,7 +69,7 @@ def test_f32_bit_exact(ctx, nstr):
     ref = rng.standard_normal((nstr, 2500)).astype(np.float32)
     ctx.upload_reference(ref)
     reads = [rng.standard_normal(n).astype(np.float32) for n in LENS[:-2]]
-    for seg in (8, 12, 16, 20, 28):
+    for seg in (8, 12, 16, 20, 25, 28, 32):  # 25..32: row costs looked ahead across steps
         sc, am = ctx.score(reads, precision="f32", segment=seg)
         osc, oam = O.batch_f32(reads, ref)
         assert np.array_equal(sc[:, :nstr], osc)
@@ -89,6 +89,44 @@ def test_reference_chunking_is_exact(ctx, prec):
     for chunk in (997, 4096, 7000):
         part = ctx.score(reads, precision=prec, ref_chunk=chunk)
         assert np.array_equal(full[0], part[0]) and np.array_equal(full[1], part[1])
+
+
+def test_renorm_chunked_items(ctx):
+    """DTWAX_F16_RENORM with the reference in chunks: every item takes its
+    bands' row minima over its own columns, so its subtracted constants, and
+    with them the binary16 rounding, depend on the chunking; each chunked
+    run equals the oracle's renormalised form on each chunk's columns (x to
+    c1) taken separately and combined by min, bit for bit, and stays within
+    rounding of the whole-strand run.  The 2100-sample read runs the split
+    schedule, its band scratch indexed from the item's first column."""
+    rng = np.random.default_rng(61)
+    N = 20000
+    ref = rng.standard_normal((2, N)).astype(np.float32)
+    ctx.upload_reference(ref)
+    lens = (5, 300, 1100, 2100)
+    reads = [rng.standard_normal(n).astype(np.float32) for n in lens]
+    reads.append(ref[0, 11000:13000] + rng.normal(0, 0.1, 2000).astype(np.float32))  # crosses 12288
+    full = ctx.score(reads, precision="f16_renorm", ref_chunk=0)
+    sh = np.stack([O.f32_to_f16(ref[k]) for k in range(2)])
+    qh = [O.f32_to_f16(r) for r in reads]
+    for chunk in (4096, 7000):
+        sc, am = ctx.score(reads, precision="f16_renorm", ref_chunk=chunk)
+        for i, q in enumerate(qh):
+            for k in range(2):
+                best = (np.float32(np.inf), -1)
+                for c0 in range(0, N, chunk):
+                    c1 = min(N, c0 + chunk)
+                    x = max(0, c0 - (len(q) - 1))
+                    # the item's DP over [x, c1) with its own renormalisation;
+                    # its minimum counts over [c0, c1) only
+                    row, off = O.sdtw_f16_renorm_row(q, sh[k, x:c1], S=8, split=len(q) > 1536)
+                    j = int(np.argmin(row[c0 - x:]))
+                    v = np.float32(row[c0 - x + j] + off)
+                    if v < best[0]:
+                        best = (v, c0 + j)
+                assert sc[i, k] == best[0] and am[i, k] == best[1], (chunk, i, k, sc[i, k], best)
+        rel = np.abs(sc - full[0]) / np.abs(full[0])
+        assert rel.max() < 5e-3, (chunk, rel.max())
 
 
 def test_exact_substring_scores_zero(ctx):
