@@ -127,7 +127,7 @@ def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
         for rep in range(3):
             s0 = ru.stats()
             t1 = time.perf_counter()
-            ru.process(np.arange(512), np.full(512, rep + 1), sig)
+            d_exact = ru.process(np.arange(512), np.full(512, rep + 1), sig)
             wall = time.perf_counter() - t1
             s1 = ru.stats()
             sat.append((wall, s1["gpu_ms"] - s0["gpu_ms"], s1["cells"] - s0["cells"]))
@@ -135,6 +135,30 @@ def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
             ru.release(c)
         wall, kms, cells = min(sat)
         print(f"[bench] read until: saturated batch {wall * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        # the opt-in fused form (one rounding of t*t + min) on the same chunks
+        ruf = D.ReadUntil(ctx, max_reads=512, chunk_len=1600, max_chunks=4, precision="f16_fused",
+                          threshold_per_row=thr, norm=mode)
+        try:
+            fsat = []
+            for rep in range(3):
+                s0 = ruf.stats()
+                t1 = time.perf_counter()
+                d_fused = ruf.process(np.arange(512), np.full(512, rep + 1), sig)
+                fsat.append((time.perf_counter() - t1, ruf.stats()["gpu_ms"] - s0["gpu_ms"]))
+            bf = None
+            if burst_seconds > 0:
+                for c in range(512):
+                    ruf.release(c)
+                bf = ru_sim.simulate_realtime(ruf, src, ru_sim.SimConfig(n_channels=512, duration_s=burst_seconds,
+                                                                         seed=2, burst=True))
+                print(f"[bench] read until: burst, fused form {bf['sim_seconds']} s", file=sys.stderr, flush=True)
+        finally:
+            ruf.close()
+        fwall, fkms = min(fsat)
+        fused_sat = {"wall_ms": round(fwall * 1e3, 2), "kernel_ms": round(fkms, 2),
+                     "kernel_gcups": round(cells / (fkms * 1e-3) / 1e9, 1),
+                     "actions_differing_from_exact": int(np.sum(d_fused["action"] != d_exact["action"]))}
+        print(f"[bench] read until: saturated batch, fused form {fwall * 1e3:.1f} ms", file=sys.stderr, flush=True)
         cfg = ru_sim.SimConfig(n_channels=512, duration_s=seconds, seed=1)
         m = ru_sim.simulate_realtime(ru, src, cfg)
         print(f"[bench] read until: served {m['sim_seconds']} s", file=sys.stderr, flush=True)
@@ -162,7 +186,8 @@ def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
                             # 3 element ops per cell against packed VALU
                             "roofline_frac": round(3.0 * cells / (kms * 1e-3) / valu_peak_ops(ctx), 4),
                             "exact_form_ceiling_ms": round(4.0 * cells / valu_peak_ops(ctx) * 1e3, 1),
-                            "chunks_per_s": round(512 / wall, 1)},
+                            "chunks_per_s": round(512 / wall, 1),
+                            "fused_form_opt_in": fused_sat},
         "plain_batched_form": plain,
         "serving_vs_plain_reads_per_s": round(m["reads_decided_per_s"] / max(plain["reads_decided_per_s"], 1e-9), 3),
         "setup_s": round(time.time() - t0, 1),
@@ -173,6 +198,8 @@ def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
                       **{k: b[k] for k in ("sim_seconds", "reads_started", "reads_decided", "reads_decided_per_s",
                                            "latency_ms_p50", "latency_ms_p99", "latency_ms_max",
                                            "decision_latency_ms_p99", "mean_batch", "accuracy")}}
+        m["burst"]["fused_form_opt_in"] = {k: bf[k] for k in ("reads_decided", "latency_ms_p50", "latency_ms_p99",
+                                                              "latency_ms_max", "accuracy")}
     return m
 
 

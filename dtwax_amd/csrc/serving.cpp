@@ -541,8 +541,10 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   if (o.max_reads < 1 || o.chunk_len < 64 || o.max_chunks < 1 || o.n_streams < 1 || o.n_streams > 8)
     return set_error(DTWAX_EINVAL, "max_reads, chunk_len (>= 64), max_chunks and n_streams (1..8) "
                                    "must be positive");
-  if (o.precision != DTWAX_F16 && o.precision != DTWAX_F32 && o.precision != DTWAX_F16_RENORM)
-    return set_error(DTWAX_EINVAL, "Read Until serving supports DTWAX_F16, DTWAX_F16_RENORM and DTWAX_F32");
+  if (o.precision != DTWAX_F16 && o.precision != DTWAX_F32 && o.precision != DTWAX_F16_RENORM &&
+      o.precision != DTWAX_F16_FUSED)
+    return set_error(DTWAX_EINVAL, "Read Until serving supports DTWAX_F16, DTWAX_F16_FUSED, DTWAX_F16_RENORM "
+                                   "and DTWAX_F32");
   if (o.norm < DTWAX_NORM_NONE || o.norm > DTWAX_NORM_MEDMAD) return set_error(DTWAX_EINVAL, "bad norm");
   if (o.chunk_len % 64) return set_error(DTWAX_EINVAL, "chunk_len must be a multiple of 64");
   const int rows_per_lane = o.chunk_len / 64;

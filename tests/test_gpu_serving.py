@@ -48,9 +48,9 @@ def _oracle_chunked(y, ref, chunk_len, precision):
         row = None
         for k in range(n_chunks):
             q = y[k * chunk_len:(k + 1) * chunk_len]
-            if precision == "f16":
+            if precision in ("f16", "f16_fused"):
                 sc, am, row = O.sdtw_f16(O.f32_to_f16(q), O.f32_to_f16(ref[st]), init_row=row,
-                                         want_row=True)
+                                         want_row=True, fused=precision == "f16_fused")
                 out[k, st] = O.f16_to_f32(np.array([sc], np.uint16))[0]
             else:
                 sc, am, row = O.sdtw_f32(q, ref[st], init_row=row, want_row=True)
@@ -87,6 +87,7 @@ def _run_staggered(ru, reads, chunk_len, n_chunks, stagger=3):
     ("f16", 1024, 1024),   # S=16
     ("f16", 3200, 4096),   # two bands per chunk (band scratch between them)
     ("f32", 1600, 2048),
+    ("f16_fused", 1600, 2048),  # the opt-in fused form
 ])
 def test_ru_chunked_matches_oracle_and_scratch(ctx, precision, chunk_len, min_ref_chunk):
     n_chunks = 4
@@ -333,7 +334,6 @@ def test_ru_admission_by_hbm_budget_and_errors(ctx):
     finally:
         ru.close()
     ctx.upload_reference(ref)
-    for kw in (dict(chunk_len=1000), dict(chunk_len=1600, segment=32), dict(precision="f16_fused"),
-               dict(n_streams=9), dict(hbm_budget=100)):
+    for kw in (dict(chunk_len=1000), dict(chunk_len=1600, segment=32), dict(n_streams=9), dict(hbm_budget=100)):
         with pytest.raises(D.DtwaxError):
             D.ReadUntil(ctx, **kw)
