@@ -629,6 +629,9 @@ def main(argv=None):
     ap.add_argument("--eject", choices=("nontarget", "target"), default="nontarget")
     ap.add_argument("--seconds", type=float, default=None)
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--precision", choices=("f16", "f16_fused", "f16_renorm", "f32"), default="f16",
+                    help="arithmetic form of the serving engine (f16: the reference's half2 ops; "
+                         "f16_fused: ~25%% more chunks per second under a full burst)")
     a = ap.parse_args(argv)
 
     target = f"{a.host}:{a.port}"
@@ -643,7 +646,7 @@ def main(argv=None):
     levels, _ = D.squiggle_build(D.read_fasta(a.fasta), means, k=k)
     ctx.upload_reference(levels)
     n = a.last_channel - a.first_channel + 1
-    ru = D.ReadUntil(ctx, max_reads=n, chunk_len=a.chunk_len, max_chunks=a.max_chunks,
+    ru = D.ReadUntil(ctx, max_reads=n, chunk_len=a.chunk_len, max_chunks=a.max_chunks, precision=a.precision,
                      norm=D.NORM_MEANSTD, threshold_per_row=a.threshold_per_row)
     client = LiveReadsClient(channel, cache_size=n)
     client.run(first_channel=a.first_channel, last_channel=a.last_channel)
