@@ -205,13 +205,16 @@ constexpr int PG = 2;
 // M3 (the reference built without -DNO_REF_DEL, DTW.cu:23-27): the left move
 // D[r][c-1] = D_i(old) joins the min, which becomes one three-input min in
 // the chain's min slot (still 4 ops per cell).
+//
+// DTWAX_F16_RENORM: row 0 (the lane's first) costs rn(t*t - K), one fused op
+// in place of the multiply (negK = -K).
 template <int S, int MODE, bool M3>
 __device__ __forceinline__ void f16_cells_pipe(const F16Band<S, MODE>& B, h2 s, h2 s_next, h2 (&tc)[PG],
-                                               h2 tin, h2 dg, const h2 (&Do)[S], h2 (&Dn)[S]) {
+                                               h2 tin, h2 dg, const h2 (&Do)[S], h2 (&Dn)[S], h2 negK) {
   h2 t[S], c[S];
 #pragma unroll
   for (int i = 0; i < PG; i++) t[i] = tc[i];
-  c[0] = t[0] * t[0];
+  c[0] = (MODE == DTWAX_F16_RENORM) ? hfma2(t[0], t[0], negK) : t[0] * t[0];
   h2 dd = dg;
   __builtin_amdgcn_sched_barrier(0);
   h2 tp = tin;
@@ -245,10 +248,15 @@ struct SweepIO {
   h2* lrow;          // FIX, final band: the last row also in the reference's tiled
                      // layout (this read's row of KParams::last_row), or null
   int32_t lr_ntiles, lr_last;
+  h2 tlow;           // DTWAX_F16_RENORM: subtracted from the top row as it is staged
+                     // (the previous band's, or the stored row's, exact minimum)
 };
 
 // Sweep one band across columns [x, c1) (+ ramp).  Accumulates the min/argmin
 // of the band's last row over [c0, c1) into best/pos when final_band.
+// DTWAX_F16_RENORM: K is the lane's drift, folded into its first row's cost
+// (f16_cells_pipe); the top row is lowered by io.tlow as it is staged;
+// rowmin receives the band's last-row minimum over [x, c1).
 template <int S, int MODE, bool M3>
 __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const SweepIO& io, h2 K,
                                           h2* __restrict__ ring_s, h2* __restrict__ ring_t,
@@ -271,6 +279,11 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
     ring_s[128 + lane] = c;
     h2 ta = ZERO, tb = ZERO, tc = ZERO;
     if (trow) { ta = trow[x - 64 + lane]; tb = trow[x + lane]; tc = trow[x + 64 + lane]; }
+    if (MODE == DTWAX_F16_RENORM) {  // the top row's lowering, as it is staged
+      ta = ta - io.tlow;
+      tb = tb - io.tlow;
+      tc = tc - io.tlow;
+    }
     ring_t[lane] = ta;
     ring_t[RING + lane] = ta;
     ring_t[64 + lane] = tb;
@@ -284,8 +297,9 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
 #pragma unroll
   for (int i = 0; i < S; i++) D0[i] = INF;
   h2 tin = INF, dg = INF, hist = INF;
-  if (lane == 0) tin = ring_t[64] - K;  // top row at column x for lane 0
-  if (lane == 0 && io.top_diag) dg = ring_t[63] - K;
+  if (lane == 0) tin = ring_t[64];  // top row at column x for lane 0
+  if (lane == 0 && io.top_diag) dg = ring_t[63];
+  const h2 negK = -K;
 
   constexpr bool PIPE = MODE != DTWAX_F16_FUSED;
   // pipelined forms: column of step 0 and its first PG row costs
@@ -299,6 +313,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
     // Stream block T+128 .. T+191 (written to the ring after this block).
     h2 pf_s = ref[x + T + 128 + lane];
     h2 pf_t = trow ? trow[x + T + 128 + lane] : ZERO;
+    if (MODE == DTWAX_F16_RENORM) pf_t = pf_t - io.tlow;
     const int base = (64 - lane + T) & (RING - 1);
 #pragma unroll 1
     for (int g = 0; g < 64; g += 8) {
@@ -313,7 +328,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
             // ahead (slot base + 65 at most: inside the mirrored ring)
             const h2 s_next = rs[u + 1];
             const h2 tf_next = rt[u + 2];
-            f16_cells_pipe<S, MODE, M3>(B, s_cur, s_next, tc, tin, dg, D0, D1);
+            f16_cells_pipe<S, MODE, M3>(B, s_cur, s_next, tc, tin, dg, D0, D1, negK);
             s_cur = s_next;
             tf = tf_cur;
             tf_cur = tf_next;
@@ -324,7 +339,6 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           }
           dg = tin;
           tin = dpp_shr1(tf, D1[S - 1]);
-          if (MODE == DTWAX_F16_RENORM) tin = tin - K;
           hist = dpp_shl1(D1[S - 1], hist);
         }
         {
@@ -332,7 +346,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           if constexpr (PIPE) {
             const h2 s_next = rs[u + 2];
             const h2 tf_next = rt[u + 3];
-            f16_cells_pipe<S, MODE, M3>(B, s_cur, s_next, tc, tin, dg, D1, D0);
+            f16_cells_pipe<S, MODE, M3>(B, s_cur, s_next, tc, tin, dg, D1, D0, negK);
             s_cur = s_next;
             tf = tf_cur;
             tf_cur = tf_next;
@@ -343,7 +357,6 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           }
           dg = tin;
           tin = dpp_shr1(tf, D0[S - 1]);
-          if (MODE == DTWAX_F16_RENORM) tin = tin - K;
           hist = dpp_shl1(D0[S - 1], hist);
         }
       }
@@ -401,13 +414,19 @@ constexpr int RING2_ALLOC = RING2 + MIRROR2;
 // set, the A column's rows k >= 2 take the next lane's B column row k - 2 of
 // this same step as their diagonal (made one iteration earlier and moved by
 // one DPP wave_shl:1) instead of their own previous column's row k - 1.
+//
+// DTWAX_F16_RENORM: the lane's first row (k = 0) costs rn(t*t - K), one fused
+// op in place of the multiply (negK = -K), which lowers the lane's rows by
+// its drift K at no extra op.
 template <int S, int MODE, bool QX = false>
 __device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 sB, h2 tinA, h2 tinB,
-                                           h2 diagA, h2 (&Bc)[S], h2& lastA, bool xdiag = false) {
+                                           h2 diagA, h2 (&Bc)[S], h2& lastA, bool xdiag = false,
+                                           h2 negK = (h2){(half_t)0.0f, (half_t)0.0f}) {
   constexpr bool FUSED = MODE == DTWAX_F16_FUSED;
+  constexpr bool RN = MODE == DTWAX_F16_RENORM;
   // cA/cB: the pair's row costs t*t (per-op form) or t (fused form)
   const h2 t0 = hfma2(sA, B.A[0], B.Qn[0]);  // == rn(s - q) for a real row
-  h2 cA = FUSED ? t0 : t0 * t0;
+  h2 cA = FUSED ? t0 : (RN ? hfma2(t0, t0, negK) : t0 * t0);
   h2 cB = hsplat((half_t)0.0f);
   h2 a1 = tinA, a2 = tinA, b2 = tinB;  // A_(k-1), A_(k-2), B_(k-2)
   __builtin_amdgcn_sched_barrier(0);
@@ -438,7 +457,7 @@ __device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 
       b2 = vB;
     }
     if (k + 1 < S) cA = FUSED ? nA : nA * nA;
-    if (k < S) cB = FUSED ? nB : nB * nB;
+    if (k < S) cB = FUSED ? nB : ((RN && k == 0) ? hfma2(nB, nB, negK) : nB * nB);
     if (k < S) {
       a2 = a1;
       a1 = vA;
@@ -474,9 +493,10 @@ constexpr int QK_PERIOD = TILE_W / 2;  // steps per tile (two columns per step)
 // One band over columns [x, c1) (+ ramp), two columns per step; band-final
 // min/argmin per strand over [c0, c1) into best/posf/posr, the band's last
 // row to io.orow.
-// DTWAX_F16_RENORM: K is the lane's drift, subtracted from both top inputs
-// at every lane hop (lane 0: from the band's top row); rowmin receives the
-// band's last-row minimum over [x, c1) (not the final band).
+// DTWAX_F16_RENORM: K is the lane's drift, folded into its first row's cost
+// (f16_cells2); the band's top row is lowered by io.tlow as it is staged;
+// rowmin receives the band's last-row minimum over [x, c1) (not the final
+// band).
 template <int S, int MODE, bool QK = false, bool FIX = false>
 __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const SweepIO& io, h2* ring_s,
                                           h2* ring_t, int lane, h2& best, int32_t& posf, int32_t& posr,
@@ -506,11 +526,13 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
     const int o = 64 * k + lane;
     if (o < 130) {
       put(ring_s, o + 128, ref[x + o]);
-      put(ring_t, o + 128, tsrc[x + o] * tmask);
+      h2 tv = tsrc[x + o] * tmask;
+      if constexpr (RN) tv = tv - io.tlow;
+      put(ring_t, o + 128, tv);
     }
   }
   // a resumed row also gives lane 0 its first diagonal, trow[x - 1] (slot 127)
-  if (io.top_diag && lane == 0) put(ring_t, 127, io.trow[x - 1]);
+  if (io.top_diag && lane == 0) put(ring_t, 127, RN ? io.trow[x - 1] - io.tlow : io.trow[x - 1]);
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
 
@@ -522,12 +544,8 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
     tinA = ring_t[128];
     tinB = ring_t[129];
     if (io.top_diag) diagA = ring_t[127];
-    if constexpr (RN) {
-      tinA = tinA - K;
-      tinB = tinB - K;
-      diagA = diagA - K;
-    }
   }
+  const h2 negK = -K;
   h2 rmin = INF;
   const int base0 = (128 - 2 * lane) & (RING2 - 1);
   h2 sA = ring_s[base0], sB = ring_s[base0 + 1];
@@ -542,6 +560,7 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
       const int32_t c = x + 2 * T + 130 + 64 * h + lane;
       pf_s[h] = ref[c];
       pf_t[h] = tsrc[c] * tmask;
+      if constexpr (RN) pf_t[h] = pf_t[h] - io.tlow;
     }
     const int base = (2 * T + 128 - 2 * lane) & (RING2 - 1);
 #pragma unroll 1
@@ -583,14 +602,10 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
         const h2 nA = rs[2 * u + 2], nB = rs[2 * u + 3];
         const h2 tfA = rt[2 * u + 2], tfB = rt[2 * u + 3];
         h2 lastA;
-        f16_cells2<S, MODE>(B, sA, sB, tinA, tinB, diagA, Bc, lastA);
+        f16_cells2<S, MODE>(B, sA, sB, tinA, tinB, diagA, Bc, lastA, false, negK);
         diagA = tinB;
         tinA = dpp_shr1(tfA, lastA);
         tinB = dpp_shr1(tfB, Bc[S - 1]);
-        if constexpr (RN) {
-          tinA = tinA - K;
-          tinB = tinB - K;
-        }
         histA = dpp_shl1(lastA, histA);
         histB = dpp_shl1(Bc[S - 1], histB);
         sA = nA;
@@ -819,14 +834,13 @@ __device__ void f16_item_split(const KParams& p, const WorkItem& it, int lane, h
     io.top_diag = false;
     io.lrow = nullptr;
     io.lr_ntiles = io.lr_last = 0;
+    io.tlow = prev_min;  // 0 for band 0 (free start)
     h2 rowmin = hsplat((half_t)__builtin_inff());
-    h2 Kl = Kb;
-    if (b > 0 && lane == 0) Kl = Kb + prev_min;
-    f16_item_band<SB, MODE, M3, false, false>(qp, nullptr, r0 + lane * SB, io, Kl, ring_s, ring_t, lane, best,
+    f16_item_band<SB, MODE, M3, false, false>(qp, nullptr, r0 + lane * SB, io, Kb, ring_s, ring_t, lane, best,
                                              posf, posr, rowmin);
-    const h2 k0 = __builtin_bit_cast(h2, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Kl), 0));
-    off_f += 63.0f * (float)Kb.x + (float)k0.x;
-    off_r += 63.0f * (float)Kb.y + (float)k0.y;
+    // subtracted on every path to the last row: 64 lane drifts and the top row's lowering
+    off_f += 64.0f * (float)Kb.x + (float)prev_min.x;
+    off_r += 64.0f * (float)Kb.y + (float)prev_min.y;
     if (!io.final_band) {
       h2 m = rowmin;
 #pragma unroll
@@ -880,16 +894,16 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
 
   h2 best = hsplat((half_t)__builtin_inff());
   int32_t posf = -1, posr = -1;
-  // F16_RENORM: every lane subtracts Kl from its top input, a per-row
-  // constant (the same for all columns), so the last row ends up lowered by
-  // the sum of the subtractions on its path; that sum is added back here in
-  // binary32.  Lane 0 of a band after the first also subtracts the exact
-  // minimum of the row the previous band left, per strand.
+  // F16_RENORM: every lane's first row is lowered by the lane's drift Kb
+  // (folded into its cost), a per-row constant (the same for all columns),
+  // so the last row ends up lowered by the sum of these on its path; that sum
+  // is added back here in binary32.  A band after the first also stages its
+  // top row lowered by that row's exact minimum, per strand.
   float off_f = 0.0f, off_r = 0.0f;
   h2 prev_min = hsplat((half_t)0.0f);
   h2 Kb = K;  // per-lane drift of the current band (adapted band to band)
   // Read Until: the engine's per-read drift, and the stored row's minimum,
-  // which lane 0 subtracts from the resumed top row; one offset for every
+  // by which the resumed top row is lowered; one offset for every
   // reference piece of the chunk (the pieces share the stored row), so no
   // band-to-band adaptation; the keys hold the lowered score
   const bool ru_rn = MODE == DTWAX_F16_RENORM && p.ru_renorm != nullptr;
@@ -936,19 +950,19 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
       io.lr_last = p.lr_last;
     }
     h2 rowmin = hsplat((half_t)__builtin_inff());
-    h2 Kl = Kb;
-    if (MODE == DTWAX_F16_RENORM && b > 0 && lane == 0 && !ru_rn) Kl = Kb + prev_min;
-    if (ru_rn && b == 0 && lane == 0) Kl = Kb + prev_min;
+    // the top row's lowering: the previous band's row minimum, or (Read
+    // Until, first band) the stored row's
+    const h2 tlow = (MODE == DTWAX_F16_RENORM && (ru_rn ? b == 0 : b > 0)) ? prev_min : hsplat((half_t)0.0f);
+    io.tlow = tlow;
     if constexpr (use_band2<S, MODE, M3>())
-      f16_band2<S, MODE, QK, FIX>(B, io, ring_s, ring_t, lane, best, posf, posr, Kl, &rowmin);
+      f16_band2<S, MODE, QK, FIX>(B, io, ring_s, ring_t, lane, best, posf, posr, Kb, &rowmin);
     else
-      f16_sweep<S, MODE, M3>(B, io, Kl, ring_s, ring_t, lane, best, posf, posr, rowmin);
+      f16_sweep<S, MODE, M3>(B, io, Kb, ring_s, ring_t, lane, best, posf, posr, rowmin);
     if (MODE == DTWAX_F16_RENORM) {
-      // lanes whose subtraction lies on the path to the last row
-      const int32_t first = 0;
-      const h2 k0 = __builtin_bit_cast(h2, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Kl), first));
-      off_f += (float)(63 - first) * (float)Kb.x + (float)k0.x;
-      off_r += (float)(63 - first) * (float)Kb.y + (float)k0.y;
+      // subtracted on every path to the last row: 64 lane drifts and the
+      // top row's lowering
+      off_f += 64.0f * (float)Kb.x + (float)tlow.x;
+      off_r += 64.0f * (float)Kb.y + (float)tlow.y;
       if (!io.final_band && !ru_rn) {
         h2 m = rowmin;
 #pragma unroll
@@ -1009,6 +1023,7 @@ __device__ void f16_pack_item(const KParams& p, const WorkItem& it, int lane, h2
   io.top_diag = false;
   io.lrow = nullptr;  // packed plans never write the tiled last row
   io.lr_ntiles = io.lr_last = 0;
+  io.tlow = hsplat((half_t)0.0f);
   h2 best = INF;
   int32_t posf = -1, posr = -1;
   {

@@ -52,9 +52,10 @@ typedef enum {
   DTWAX_F16 = 0,
   /* binary16 with one rounding of t*t + min (fused multiply-add). */
   DTWAX_F16_FUSED = 1,
-  /* binary16 per op, with per-row drift subtraction at every lane boundary
-   * and exact row-minimum subtraction between bands; the offsets are added
-   * back in binary32.  The row minimum is taken over a work item's own
+  /* binary16 per op, with a per-row drift taken off at every lane boundary
+   * (the lane's first row costs rn(t*t - drift), one rounding) and the exact
+   * row minimum taken off between bands; the offsets are added back in
+   * binary32.  The row minimum is taken over a work item's own
    * columns, so with the reference in chunks (ref_chunk, or automatic
    * chunking of small batches and long references) the subtracted constants
    * and the binary16 rounding depend on the chunking: results are exact for
@@ -91,7 +92,7 @@ typedef struct {
   int32_t ref_chunk;     /* reference columns per work item; 0 = automatic (whole strand when
                             it fits); chunks overlap by (query length - 1) columns, exact for
                             the no-deletion recurrence */
-  float renorm_drift;    /* DTWAX_F16_RENORM: initial per-row drift subtracted at lane
+  float renorm_drift;    /* DTWAX_F16_RENORM: initial per-row drift taken off at lane
                             boundaries (adapted band to band); <= 0 = default 0.01 */
   int32_t compat_quirks; /* must be 0: the reference kernel's seam deviations exist only
                             in its fixed geometry, reached through dtwax_distances */
@@ -410,7 +411,7 @@ typedef struct {
                               threshold_per_row * k * chunk_len (replaces threshold) */
   int32_t n_streams;     /* serving loop: batches in flight at once, each on its own HIP
                             stream (1..8; 0: 2) */
-  float renorm_drift;    /* DTWAX_F16_RENORM: initial per-row drift subtracted at lane hops,
+  float renorm_drift;    /* DTWAX_F16_RENORM: initial per-row drift taken off at lane boundaries,
                             adapted chunk to chunk (<= 0: 0.01) */
   int32_t reserved;
 } dtwax_ru_opts;

@@ -8,16 +8,17 @@
  * per-op cell t = rn(s - q), t2 = rn(t*t), D = rn(t2 + min(top, diag)).
  * The query's rows run in bands of 64*S rows (S rows per lane of a 64-lane
  * wave), the first band front-padded with zero-cost virtual rows so the
- * last row ends a band.  At every lane boundary (every S rows) the input row
- * is lowered by the lane's drift: a[c] = rn(a[c] - K_L) for every column
- * (column -1 stays +inf), so both the top and the diagonal input of the
- * lane's first row are lowered.  K_L = Kb, except lane 0 of a band after the
- * first, which also removes the exact minimum of the row the previous band
- * left: K_0 = rn(Kb + prev_min).  Kb starts at rn(drift * S) and after each
- * band but the last becomes rn(Kb + rn(m * 2^-6)), m = that band's last-row
+ * last row ends a band.  Every lane lowers its rows by the lane's drift Kb,
+ * folded into the cost of its first row (every S-th row of the band):
+ * t2 = rn(t*t - Kb), one rounding (a fused multiply-add), in place of
+ * rn(t*t); virtual rows have t = 0, so that row costs -Kb.  A band after
+ * the first also lowers its top row, the row the previous band left, by
+ * that row's exact minimum m as it is staged: a[c] = rn(a[c] - m) for every
+ * column (column -1 stays +inf).  Kb starts at rn(drift * S) and after each
+ * band but the last becomes rn(Kb + rn(m * 2^-6)), m that band's last-row
  * minimum (the drift it left behind, over 64 lane hops).  The amounts are
- * added back in binary32, band by band: off += 63 * Kb + K_0; the score is
- * (float)min(last row) + off, at the lowest column of the minimum.
+ * added back in binary32, band by band: off += (64 * Kb + m_prev); the score
+ * is (float)min(last row) + off, at the lowest column of the minimum.
  *
  * The split schedule (orc_sdtw_f16_renorm_split, the automatic choice for
  * reads over 1536 rows): the first 513..1536 rows in bands of 64 x 8 rows,
@@ -26,9 +27,10 @@
  * after at most 512 rows, and the long bands start from an adapted drift.
  *
  * Parity pinned by: with drift 0 and one band it equals orc_sdtw_f16; an
- * independent per-cell numpy float16 restatement on small inputs with
- * several bands (tests/test_oracle_renorm.py); the GPU kernels of
- * DTWAX_F16_RENORM bit for bit at every segment width (tests/test_gpu_parity.py).
+ * independent per-cell restatement (numpy float16, the fused first-row cost
+ * in exact rationals) on small inputs with several bands
+ * (tests/test_oracle_renorm.py); the GPU kernels of DTWAX_F16_RENORM bit for
+ * bit at every segment width (tests/test_gpu_parity.py).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -65,19 +67,18 @@ static void renorm_schedule(const uint16_t* q, int64_t Q, const uint16_t* s, int
     const int Sb = second ? S2 : S;
     const int64_t BRb = 64 * (int64_t)Sb;
     if (band == nb1 && band > 0) Kb = f2h(2.0f * h2f(Kb)); /* the drift of twice the rows per lane */
-    const uint16_t K0 = band > 0 ? f2h(h2f(Kb) + h2f(prev_min)) : Kb;
+    const uint16_t low = band > 0 ? prev_min : 0;
+    if (band > 0)
+      for (int64_t c = 0; c < N; c++) a[c + 1] = f2h(h2f(a[c + 1]) - h2f(low));
+    const uint16_t negK = (uint16_t)(Kb ^ 0x8000u);
     for (int64_t i = 0; i < BRb; i++) {
-      if (i % Sb == 0) {
-        const uint16_t K = (i == 0) ? K0 : Kb;
-        const float kf = h2f(K);
-        for (int64_t c = 0; c < N; c++) a[c + 1] = f2h(h2f(a[c + 1]) - kf);
-      }
       const int64_t r = second ? Q1 + (band - nb1) * BR2 + i : band * BR - P + i;
       const float qr = r >= 0 ? h2f(q[r]) : 0.0f;
+      const int lane_first = (i % Sb) == 0;
       for (int64_t c = 0; c < N; c++) {
         const uint16_t t = r >= 0 ? f2h(h2f(s[c]) - qr) : 0;
         const float tf = h2f(t);
-        const uint16_t t2 = f2h(tf * tf);
+        const uint16_t t2 = lane_first ? orc_h_fma(t, t, negK) : f2h(tf * tf);
         const uint16_t m = hmin(a[c + 1], a[c]);
         b[c + 1] = f2h(h2f(t2) + h2f(m));
       }
@@ -85,9 +86,10 @@ static void renorm_schedule(const uint16_t* q, int64_t Q, const uint16_t* s, int
       a = b;
       b = tmp;
     }
-    /* the amounts this band subtracted, in binary32 as the kernel adds them */
-    const float t1 = 63.0f * h2f(Kb);
-    const float t2 = t1 + h2f(K0);
+    /* the amounts this band subtracted on every path, in binary32 as the
+     * kernel adds them */
+    const float t1 = 64.0f * h2f(Kb);
+    const float t2 = t1 + h2f(low);
     off = off + t2;
     if (band + 1 < nb) {
       uint16_t m = HINF;

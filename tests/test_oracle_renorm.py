@@ -2,15 +2,19 @@
 
 * With drift 0 and a query of at most one band it is the plain binary16
   recurrence (orc_sdtw_f16) exactly: lowering by 0 changes nothing.
-* An independent per-cell restatement with numpy float16 scalars (lane
-  boundaries every S rows, lowered top and diagonal inputs, the band-to-band
-  drift rule, the binary32 offsets) gives the same score and argmin bit for
-  bit on small inputs with several bands; a plausible slip (no lane-0
-  minimum subtraction, or lowering only the top input) changes the result.
+* An independent per-cell restatement with numpy float16 scalars (the
+  lane's first row every S rows costing rn(t*t - K) with one rounding,
+  taken from exact rationals; a band's top row lowered by its minimum; the
+  band-to-band drift rule; the binary32 offsets) gives the same score and
+  argmin bit for bit on small inputs with several bands; a plausible slip
+  (t*t rounded before the drift comes off, or no top-row lowering) changes
+  the result.
 * The split schedule (orc_sdtw_f16_renorm_split: 64 x 8-row bands, then
   64 x 16-row bands at twice the drift) against the same restatement.
 * It stays within 2% of FP64 where plain binary16 drifts several percent.
 """
+from fractions import Fraction
+
 import numpy as np
 import pytest
 
@@ -32,7 +36,16 @@ def _bands(Q, S, split):
     return out
 
 
-def _np_renorm(qh, sh, S, drift=0.01, lower_diag=True, lane0_min=True, split=False, double=True):
+def _rn16(x):
+    """Exact rational x rounded to binary16 (nearest, ties to even)."""
+    h = np.float16(float(x))
+    if not np.isfinite(h):
+        return h
+    cands = [np.nextafter(h, H(-np.inf)), h, np.nextafter(h, H(np.inf))]
+    return min(cands, key=lambda v: (abs(Fraction(float(v)) - x), int(np.array(v).view(np.uint16)) & 1))
+
+
+def _np_renorm(qh, sh, S, drift=0.01, fold=True, top_min=True, split=False, double=True):
     Q, N = len(qh), len(sh)
     bands = _bands(Q, S, split)
     nb = len(bands)
@@ -42,25 +55,22 @@ def _np_renorm(qh, sh, S, drift=0.01, lower_diag=True, lane0_min=True, split=Fal
     for band, (Sb, rb) in enumerate(bands):
         if double and band > 0 and Sb != bands[band - 1][0]:
             Kb = H(Kb + Kb)  # twice the rows per lane
-        K0 = H(Kb + prev_min) if (band > 0 and lane0_min) else Kb
+        low = prev_min if (band > 0 and top_min) else H(0)
+        if band > 0:  # the top row, staged lowered by its minimum
+            a = np.array([H(v - low) for v in a], H)
         for i in range(64 * Sb):
-            if i % Sb == 0:
-                K = K0 if i == 0 else Kb
-                lowered = np.array([H(v - K) for v in a], H)
-                top = lowered
-                dg_row = lowered if lower_diag else a
-            else:
-                top = dg_row = a
             r = rb + i
             new = np.empty(N, H)
             for c in range(N):
                 t = H(sh[c] - qh[r]) if r >= 0 else H(0)
-                t2 = H(t * t)
-                dg = dg_row[c - 1] if c > 0 else inf
-                m = min(top[c], dg)
-                new[c] = H(t2 + m)
+                if i % Sb == 0:  # the lane's first row: rn(t*t - Kb), one rounding
+                    t2 = _rn16(Fraction(float(t)) ** 2 - Fraction(float(Kb))) if fold else H(H(t * t) - Kb)
+                else:
+                    t2 = H(t * t)
+                dg = a[c - 1] if c > 0 else inf
+                new[c] = H(t2 + min(a[c], dg))
             a = new
-        off = np.float32(off + np.float32(np.float32(63) * np.float32(Kb) + np.float32(K0)))
+        off = np.float32(off + np.float32(np.float32(64) * np.float32(Kb) + np.float32(low)))
         if band + 1 < nb:
             m = a.min()
             prev_min = m
@@ -79,7 +89,9 @@ def test_drift_zero_one_band_is_plain_f16():
         assert sc == O.f16_to_f32(np.array([p], np.uint16))[0] and am == pa, (S, Q)
 
 
-@pytest.mark.parametrize("S,Q,N,seed", [(8, 700, 37, 1), (8, 1100, 23, 2), (12, 900, 31, 3)])
+# seeds on which each slip below changes the score (rounding t*t before the
+# drift comes off changes few cells, so not every input shows it)
+@pytest.mark.parametrize("S,Q,N,seed", [(8, 700, 37, 17), (8, 1100, 23, 1), (12, 900, 31, 11)])
 def test_per_cell_numpy_restatement(S, Q, N, seed):
     rng = np.random.default_rng(seed)
     qh = (rng.standard_normal(Q) * 1.5).astype(H)
@@ -87,9 +99,10 @@ def test_per_cell_numpy_restatement(S, Q, N, seed):
     want, wam = _np_renorm(qh, sh, S)
     got, gam = O.sdtw_f16_renorm(qh.view(np.uint16), sh.view(np.uint16), S)
     assert np.float32(got) == want and gam == wam, (got, want, gam, wam)
-    # plausible slips give a different number
-    slips = [_np_renorm(qh, sh, S, lower_diag=False)[0], _np_renorm(qh, sh, S, lane0_min=False)[0]]
-    assert all(sl != want for sl in slips), slips
+    # plausible slips give a different number: rounding t*t before the
+    # drift is taken off, or no lowering of a band's top row
+    slips = [_np_renorm(qh, sh, S, fold=False)[0], _np_renorm(qh, sh, S, top_min=False)[0]]
+    assert all(sl != want for sl in slips), (slips, want)
 
 
 def test_split_schedule_per_cell():
