@@ -434,9 +434,9 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   // 64 x 8 rows (two or more bands); longer ones run the S = 16 kernel's
   // split schedule (their first 513..1536 rows in bands of 64 x 8 rows, the
   // rest in bands of 64 x 16 rows on the two-column sweep).  Against FP64
-  // (max relative error of min(fwd, rev), 48 reads of the 48.5 kb workload):
-  // 2000 rows at S = 8 0.36%, split 0.46%, S = 16 throughout 1.25%; 3000
-  // rows 0.52%, 0.46%, 0.86%.
+  // (max relative error of min(fwd, rev), 1024 reads of the 48.5 kb
+  // workload): 0.75% / 0.58% / 0.55% / 0.72% at 1000 / 2000 / 4000 / 8000
+  // rows; a first band of 1024 rows instead gave 1.8% at 2000 rows.
   std::vector<int> seg_of((size_t)n_reads, 0);
   if (o.segment) {
     std::fill(seg_of.begin(), seg_of.end(), o.segment);
