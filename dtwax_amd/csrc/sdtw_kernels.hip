@@ -571,20 +571,26 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
         int32_t r0 = tmod + g;
         if (r0 >= QK_PERIOD) r0 -= QK_PERIOD;
         if (r0 == 28 || r0 == 32 || r0 == 44 || r0 == 60) {
-          // a group holding a deviating step (uniform across the wave)
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
+          // a group holding a deviating step (uniform across the wave).  Only
+          // steps 30 and 62 (u = 2 of groups 28 and 60) need the cross-lane
+          // diagonal (f16_cells2's QX form, S - 2 more lane moves); the
+          // others change one lane's first diagonal, a select
+          const bool xgroup = r0 == 28 || r0 == 60;
+          auto step = [&](auto uc, auto qx) {
+            constexpr int u = decltype(uc)::value;
             const int32_t t = T + g + u, r = r0 + u;
             const bool q1 = (r == 32 && t >= QK_PERIOD + 32) || r == 45;      // lane 32
             const bool q2 = (r == 30 || r == 31 || r == 62 || r == 63) && t >= 2 * QK_PERIOD + r;
             h2 dA = diagA;
             if (q1 && lane == 32) dA = INF;
             if (q2 && (r & 1) && lane == r) dA = Bc[S - 2];                  // lanes 31, 63: own row 30
-            const bool xdiag = q2 && !(r & 1) && lane == r;                  // lanes 30, 62
             const h2 nA = rs[2 * u + 2], nB = rs[2 * u + 3];
             const h2 tfA = rt[2 * u + 2], tfB = rt[2 * u + 3];
             h2 lastA;
-            f16_cells2<S, MODE, true>(B, sA, sB, tinA, tinB, dA, Bc, lastA, xdiag);
+            if constexpr (decltype(qx)::value)
+              f16_cells2<S, MODE, true>(B, sA, sB, tinA, tinB, dA, Bc, lastA, xgroup && q2 && lane == r);  // lanes 30, 62
+            else
+              f16_cells2<S, MODE>(B, sA, sB, tinA, tinB, dA, Bc, lastA);
             diagA = tinB;
             tinA = dpp_shr1(tfA, lastA);
             tinB = dpp_shr1(tfB, Bc[S - 1]);
@@ -592,7 +598,12 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
             histB = dpp_shl1(Bc[S - 1], histB);
             sA = nA;
             sB = nB;
-          }
+          };
+          using F = std::false_type;
+          step(std::integral_constant<int, 0>{}, F{});
+          step(std::integral_constant<int, 1>{}, F{});
+          step(std::integral_constant<int, 2>{}, std::true_type{});
+          step(std::integral_constant<int, 3>{}, F{});
           continue;
         }
       }
