@@ -17,9 +17,11 @@
 // pass between lanes with one DPP wave_shr:1 per step (the lane above's last
 // cell becomes this lane's top input; last step's top input is the diagonal).
 // The last row of the band (lane 63, register S-1) is gathered over 64 steps
-// into a history register with one DPP wave_shl:1 per step and then flushed:
-// to the running min/argmin (final band) or to a row buffer (next band, or
-// the Read Until state).
+// in an LDS strip (every lane stores its last cell at slot k + 63 - lane at
+// step k of a block, so lane 63's store is the last one at slots 0..63; a
+// store rides the LDS pipe, where a DPP move would take a VALU issue slot)
+// and then flushed: to the running min/argmin (final band) or to a row
+// buffer (next band, or the Read Until state).
 //
 // Reference columns and the top-boundary row are streamed HBM -> LDS ring
 // (256 slots + an 80-slot mirror so a lane reads the 64 consecutive slots of
@@ -146,6 +148,12 @@ __device__ __forceinline__ int64_t tiled_index(int32_t ntiles, int32_t last, int
 constexpr int RING = 256;        // ring slots
 constexpr int MIRROR = 80;       // slots [0, MIRROR) are also kept at RING + slot
 constexpr int RING_ALLOC = RING + MIRROR;
+// Last-row strip of a sweep, in LDS after the top-row ring: at step k of a
+// 64-step block lane L stores its last cell at slot k + 63 - L (slots 0..126);
+// lane 63 stores slot k at step k after every other lane's store there, so
+// after the block slot j holds the last row at the block's j-th column.  One
+// strip per column of a step (the two-column sweep has two).
+constexpr int HSTRIP = 128;
 
 // ------------------------------------------------------------------------
 // binary16 forms (DTWAX_F16, DTWAX_F16_FUSED, DTWAX_F16_RENORM)
@@ -296,7 +304,9 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
   h2 D0[S], D1[S];
 #pragma unroll
   for (int i = 0; i < S; i++) D0[i] = INF;
-  h2 tin = INF, dg = INF, hist = INF;
+  h2 tin = INF, dg = INF;
+  h2* const hstrip = ring_t + RING_ALLOC;  // the last-row strip
+  h2* const hput = hstrip + (63 - lane);
   if (lane == 0) tin = ring_t[64];  // top row at column x for lane 0
   if (lane == 0 && io.top_diag) dg = ring_t[63];
   const h2 negK = -K;
@@ -339,7 +349,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           }
           dg = tin;
           tin = dpp_shr1(tf, D1[S - 1]);
-          hist = dpp_shl1(D1[S - 1], hist);
+          hput[g + u] = D1[S - 1];
         }
         {
           h2 tf;
@@ -357,11 +367,12 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
           }
           dg = tin;
           tin = dpp_shr1(tf, D0[S - 1]);
-          hist = dpp_shl1(D0[S - 1], hist);
+          hput[g + u + 1] = D0[S - 1];
         }
       }
     }
-    // hist lane j = last-row value of column x + T + j - 63
+    // strip slot j = last-row value of column x + T + j - 63
+    const h2 hist = hstrip[lane];
     const int32_t col = x + T + lane - 63;
     const bool in_item = (col >= x) && (col < io.c1);
     if (io.final_band) {
@@ -433,6 +444,19 @@ __device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 
 #pragma unroll
   for (int k = 0; k <= S; k++) {
     h2 mA, mB, nA, nB;
+    auto row_costs = [&]() {
+      if (k + 1 < S) nA = hfma2(sA, B.A[k + 1], B.Qn[k + 1]);
+      if (k < S) nB = hfma2(sB, B.A[k], B.Qn[k]);
+    };
+    // The first mins take the top inputs that the lane move just delivered.
+    // In the per-op forms from S = 25 up the row costs go first, so the move
+    // has time to land (S = 32: 44.7 -> 44.0 ms on the 48.5 kb x 4096 x 2000
+    // batch); the fused form and narrower bands run faster without it.
+    constexpr bool COSTS_FIRST = !FUSED && S >= 25;
+    if (COSTS_FIRST && k == 0) {
+      row_costs();
+      __builtin_amdgcn_sched_barrier(0);
+    }
     if constexpr (QX) {
       if (k < S) {
         h2 d = k == 0 ? diagA : Bc[k - 1];
@@ -446,8 +470,7 @@ __device__ __forceinline__ void f16_cells2(const F16Band<S, MODE>& B, h2 sA, h2 
       if (k < S) mA = hmin2(a1, k == 0 ? diagA : Bc[k - 1]);  // A_k: top A_(k-1), diagonal Bprev_(k-1)
     }
     if (k >= 1) mB = hmin2(b2, a2);                          // B_(k-1): top B_(k-2), diagonal A_(k-2)
-    if (k + 1 < S) nA = hfma2(sA, B.A[k + 1], B.Qn[k + 1]);
-    if (k < S) nB = hfma2(sB, B.A[k], B.Qn[k]);
+    if (!(COSTS_FIRST && k == 0)) row_costs();
     __builtin_amdgcn_sched_barrier(0);
     h2 vA;
     if (k < S) vA = FUSED ? hfma2(cA, cA, mA) : cA + mA;
@@ -539,7 +562,12 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
   h2 Bc[S];
 #pragma unroll
   for (int i = 0; i < S; i++) Bc[i] = INF;
-  h2 tinA = INF, tinB = INF, diagA = INF, histA = INF, histB = INF;
+  h2 tinA = INF, tinB = INF, diagA = INF;
+  // the last-row strips of the A and B columns
+  h2* const hstripA = ring_t + RING2_ALLOC;
+  h2* const hstripB = hstripA + HSTRIP;
+  h2* const hputA = hstripA + (63 - lane);
+  h2* const hputB = hstripB + (63 - lane);
   if (lane == 0) {
     tinA = ring_t[128];
     tinB = ring_t[129];
@@ -594,8 +622,8 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
             diagA = tinB;
             tinA = dpp_shr1(tfA, lastA);
             tinB = dpp_shr1(tfB, Bc[S - 1]);
-            histA = dpp_shl1(lastA, histA);
-            histB = dpp_shl1(Bc[S - 1], histB);
+            hputA[g + u] = lastA;
+            hputB[g + u] = Bc[S - 1];
             sA = nA;
             sB = nB;
           };
@@ -617,13 +645,14 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
         diagA = tinB;
         tinA = dpp_shr1(tfA, lastA);
         tinB = dpp_shr1(tfB, Bc[S - 1]);
-        histA = dpp_shl1(lastA, histA);
-        histB = dpp_shl1(Bc[S - 1], histB);
+        hputA[g + u] = lastA;
+        hputB[g + u] = Bc[S - 1];
         sA = nA;
         sB = nB;
       }
     }
-    // lane j holds the last row at columns x + 2(T + j) - 126 (A) and + 1 (B)
+    // strip slot j holds the last row at columns x + 2(T + j) - 126 (A) and + 1 (B)
+    const h2 histA = hstripA[lane], histB = hstripB[lane];
     const int32_t colA = x + 2 * (T + lane) - 126;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -1003,7 +1032,7 @@ template <int S, int MODE, bool M3, bool QK = false, bool FIX = false>
 __global__ __launch_bounds__(64) void sdtw_f16_kernel(KParams p) {
   constexpr int RA = use_band2<S, MODE, M3>() ? RING2_ALLOC : RING_ALLOC;
   __shared__ h2 ring_s[RA];
-  __shared__ h2 ring_t[RA];
+  __shared__ h2 ring_t[RA + 2 * HSTRIP];  // + the sweeps' last-row strips
   const int lane = threadIdx.x;
   h2* scratch = (h2*)p.scratch + (int64_t)blockIdx.x * p.scratch_stride + SCRATCH_PADL;
   for (;;) {
@@ -1101,7 +1130,7 @@ __device__ void f16_pack_item(const KParams& p, const WorkItem& it, int lane, h2
 template <int S, int MODE>
 __global__ __launch_bounds__(64) void sdtw_f16_pack_kernel(KParams p) {
   __shared__ h2 ring_s[RING2_ALLOC];
-  __shared__ h2 ring_t[RING2_ALLOC];
+  __shared__ h2 ring_t[RING2_ALLOC + 2 * HSTRIP];  // + f16_band2's last-row strips
   const int lane = threadIdx.x;
   h2* scratch_w = p.scratch ? (h2*)p.scratch + (int64_t)blockIdx.x * p.scratch_rows * p.scratch_stride + SCRATCH_PADL
                             : nullptr;

@@ -272,7 +272,8 @@ int dtwax_plan_kernel_ms(dtwax_plan* plan, float* ms);
  * least 6 work items per resident wave (steady state, as in the full run).
  * It times segment in {0 = per-read widths, 8, 12, 16, 20, 25, 28, 32} ({0, 8} for
  * DTWAX_F16_RENORM) at ref_chunk min(65536, sample_cols); the fastest per
- * counted cell wins.  ref_chunk is then the plan's own rule for the whole
+ * counted cell wins, except that segment 0 (the plan's own per-read choice)
+ * stays unless a fixed width is at least 2% faster.  ref_chunk is then the plan's own rule for the whole
  * workload at that width (whole strands, or the longest chunks whose band
  * scratch fits the budget: a quarter of free device memory, at most 32 GB),
  * not a sample timing, which would favour short chunks.  A nonzero

@@ -315,8 +315,12 @@ def test_autotune_choice_cached_and_exact(ctx, tmp_path):
     best, table = ctx.autotune(d_q, d_off, lns, sample_reads=16, sample_cols=8192, cache_path=cache)
     assert len(table) == 8  # widths 0 (per read), 8, 12, 16, 20, 25, 28, 32
     assert best["segment"] in (0, 8, 12, 16, 20, 25, 28, 32) and best["ref_chunk"] > 0
+    # the fastest candidate wins, except that per-read widths (segment 0)
+    # stay unless a fixed width is at least 2% faster on the sample
     top = table[np.argmax(table["gcups"])]
-    assert top["segment"] == best["segment"]
+    g0 = float(table[table["segment"] == 0]["gcups"][0])
+    want = top["segment"] if top["gcups"] >= 1.02 * g0 else 0
+    assert best["segment"] == want
     # the chunk is the plan's own rule for the whole workload at that width
     rule = ctx.plan(lns, precision="f16", segment=best["segment"])
     assert rule.info()["ref_chunk"] == best["ref_chunk"]
