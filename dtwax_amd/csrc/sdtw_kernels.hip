@@ -405,17 +405,20 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
 // This removes the fused form's wait states (its chain min -> fma has no
 // third op per cell to put between links): 21.5 -> 23.1 T cells/s at S = 32
 // on the 48.5 kb x 4096 x 2000 batch, 20.9 -> 22.4 at S = 16; the per-op form
-// runs at S = 32 as before (17.9) and 4.5% faster at S = 16.  At S = 8 the
-// doubled per-step overhead (four lane moves per 16 cells, a 127-step ramp
-// per band) outweighs it (15.9 -> 14.4), so S = 8 keeps the one-column
-// sweep, as does the three-move recurrence.  The renormalised form lowers
-// each lane hop's two top inputs by the lane's drift (two packed
-// subtractions per step, where the one-column sweep has one) and takes the
-// band's last-row minimum in the history flush; its cells and their
-// operations are the one-column sweep's, so the results are identical.
+// runs at S = 32 as before (17.9) and 4.5% faster at S = 16.  Below S = 16
+// the per-op and fused forms keep the one-column sweep (S = 8: 49.3 against
+// 54.4 ms on the same batch, fused 39.7 against 41.8), as does the
+// three-move recurrence.  The renormalised form, whose lane drift is folded
+// into each lane's first-row cost (a fused op that the one-column step
+// issues right before its first add), runs the two-column sweep from S = 8
+// up (S = 8: 49.5 -> 48.6 ms; its split schedule 47.1 -> 46.6) and takes the
+// band's last-row minimum in the history flush.  Every form's cells and
+// their operations are the one-column sweep's, so the results are identical.
 template <int S, int MODE, bool M3>
 constexpr bool use_band2() {
-  return S >= 16 && !M3 && (MODE == DTWAX_F16 || MODE == DTWAX_F16_FUSED || MODE == DTWAX_F16_RENORM);
+  if (M3) return false;
+  if (MODE == DTWAX_F16_RENORM) return S >= 8;
+  return S >= 16 && (MODE == DTWAX_F16 || MODE == DTWAX_F16_FUSED);
 }
 constexpr int RING2 = 512;    // slots; column offset o = c - x lives in slot (o + 128) & 511
 constexpr int MIRROR2 = 136;  // slots [0, MIRROR2) are also kept at RING2 + slot
@@ -835,9 +838,9 @@ __device__ __forceinline__ void f16_item_band(const float* __restrict__ qp, cons
 
 // The renormalised form at S = 16 with the split schedule (KParams::
 // renorm_split; whole reads, not Read Until): the first
-// Q1 = Q - renorm_split_rows(Q) rows in front-padded bands of 64 x 8 rows on
-// the one-column sweep from drift renorm_k, then bands of 64 x 16 rows on
-// the two-column sweep at twice the adapted drift.  Each band's cells and
+// Q1 = Q - renorm_split_rows(Q) rows in front-padded bands of 64 x 8 rows
+// from drift renorm_k, then bands of 64 x 16 rows at twice the adapted drift
+// (both on the two-column sweep).  Each band's cells and
 // renormalisation are those of a band of its width in f16_item.
 template <int MODE, bool M3>
 __device__ void f16_item_split(const KParams& p, const WorkItem& it, int lane, h2* ring_s, h2* ring_t,
