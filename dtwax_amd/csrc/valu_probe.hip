@@ -11,7 +11,7 @@
 // CU-cycle at the device's nominal clock, like the roofline.
 //
 // The timed loops touch no memory; each lane stores one float at the end.
-// Usage: valu_probe [iterations]   (default 4000; prints a table and one
+// Usage: valu_probe [iterations]   (default 20000; prints a table and one
 // JSON line)
 #include <hip/hip_runtime.h>
 #include <stdio.h>
@@ -80,7 +80,7 @@ typedef void (*probe_fn)(int, float*);
   } while (0)
 
 int main(int argc, char** argv) {
-  const int iters = argc > 1 ? atoi(argv[1]) : 4000;
+  const int iters = argc > 1 ? atoi(argv[1]) : 20000;
   if (iters < 1 || iters > 1000000) {
     fprintf(stderr, "valu_probe: iterations must be 1..1000000\n");
     return 2;
