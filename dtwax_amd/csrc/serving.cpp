@@ -125,6 +125,8 @@ struct dtwax_ru {
   int64_t row_stride = 0;  // columns per state row, pads included
   int64_t epc = 1;         // 4-byte elements per column
   int64_t grid_slots = 0;  // resident waves of the sDTW kernel
+  int64_t n_simd = 0;      // SIMDs of the device (4 per CU)
+  int64_t piece_floor = 0; // shortest piece of the one-to-three-items-per-SIMD layouts
   int64_t items_cap = 0;
   int64_t chunk_cap = 0;   // largest reference piece (columns)
   bool multiband = false;
@@ -211,16 +213,52 @@ void ru_free(dtwax_ru* r) {
   delete r;
 }
 
-// Reference piece for a batch of m chunks: about 8 work items per resident
-// wave, never shorter than min_ref_chunk (the Q-1 overlap costs Q/piece).
+// Reference piece for a batch of m chunks.  Every piece of a read repeats
+// the Q - 1 columns before it (the overlap that makes pieces independent),
+// and the batch ends with its slowest SIMD.  Candidates:
+//  - about 8 work items per resident wave, no shorter than min_ref_chunk
+//    (load balance for large batches);
+//  - pieces that give every SIMD one, two or three items (small batches: one
+//    sDTW wave alone on a SIMD runs almost as fast as three sharing it).
+// The piece with the least modelled time of the slowest SIMD wins: items
+// per wave x waves per SIMD x (piece + Q - 1) / throughput of that many
+// waves.  Relative SIMD throughput at 1, 2, 3 waves, measured with one
+// 1600-sample chunk against a 5 Mb reference: 1.0, 1.09, 1.2 (one item per
+// SIMD, 4,928-column pieces: 1.38 ms; 782 items of 6,400 columns: 1.69 ms;
+// 1,221 items of 4,096: 2.21 ms).  A per-SIMD layout has one item per wave
+// and no slack for imbalance, so it must model 5% faster than the
+// throughput layout: at 512 chunks it modelled 1.4% faster and ran 3%
+// slower (454 against 441 ms); at 1..64 chunks it runs 7-22% faster.
 int64_t piece_for(const dtwax_ru* r, int64_t m) {
   const int64_t N = r->n_cols;
-  const int64_t want_items = 8 * r->grid_slots;
-  int64_t piece = (N * m + want_items - 1) / want_items;
-  piece = std::max<int64_t>(piece, r->o.min_ref_chunk);
-  piece = std::min<int64_t>(piece, r->chunk_cap);
-  piece = (piece + 63) / 64 * 64;
-  return std::max<int64_t>(1, std::min<int64_t>(piece, N));
+  const int64_t L = r->o.chunk_len;
+  const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(r->chunk_cap, N));
+  auto fit = [&](int64_t p, int64_t floor) {
+    p = std::max<int64_t>(p, floor);
+    p = (p + 63) / 64 * 64;
+    return std::max<int64_t>(1, std::min<int64_t>(p, cap));
+  };
+  const int64_t wmax = std::max<int64_t>(1, std::min<int64_t>(3, r->grid_slots / std::max<int64_t>(1, r->n_simd)));
+  auto model = [&](int64_t p) {
+    static const double kRate[4] = {0.0, 1.0, 1.09, 1.2};
+    const int64_t items = m * ((N + p - 1) / p);
+    const int64_t waves = std::min<int64_t>(items, r->grid_slots);
+    const int64_t per_wave = (items + waves - 1) / waves;
+    const int64_t w = std::min<int64_t>(wmax, (waves + r->n_simd - 1) / r->n_simd);
+    return (double)per_wave * (double)w * (double)(std::min<int64_t>(p, N) + L - 1) / kRate[w];
+  };
+  int64_t best = fit((N * m + 8 * r->grid_slots - 1) / (8 * r->grid_slots), r->o.min_ref_chunk);
+  double t_best = model(best);
+  t_best *= 0.95;
+  for (int64_t k = 1; k <= wmax; k++) {
+    const int64_t p = fit((N * m + k * r->n_simd - 1) / (k * r->n_simd), r->piece_floor);
+    const double t = model(p);
+    if (t < t_best) {
+      t_best = t;
+      best = p;
+    }
+  }
+  return best;
 }
 
 void set_decision(dtwax_ru_decision& d, int32_t channel, const ChanRec& c, int32_t action) {
@@ -557,6 +595,9 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   if (!valid_segment(o.segment) || rows_per_lane % o.segment)
     return set_error(DTWAX_EINVAL, "segment must be 8, 12, 16, 20, 25, 28 or 32 and divide chunk_len / 64");
   if (o.min_ref_chunk < 0 || o.hbm_budget < 0) return set_error(DTWAX_EINVAL, "negative size");
+  // a given min_ref_chunk bounds every layout; by default the per-SIMD
+  // layouts of small batches may go down to 2 x chunk_len
+  const int64_t piece_floor = o.min_ref_chunk ? o.min_ref_chunk : 2 * (int64_t)o.chunk_len;
   if (o.min_ref_chunk == 0) o.min_ref_chunk = 4 * o.chunk_len;
   if (!(o.threshold > 0.0f)) o.threshold = INFINITY;  // no early non-target decision
   if (!(o.renorm_drift > 0.0f)) o.renorm_drift = 0.01f;
@@ -585,12 +626,16 @@ int dtwax_ru_create(dtwax_ctx* c, const dtwax_ru_opts* opts, dtwax_ru** out) {
   };
   if (hipSetDevice(c->device) != hipSuccess) return cleanup(DTWAX_EDEVICE, "hipSetDevice");
   r->grid_slots = (int64_t)std::max(1, occupancy(o.precision, r->S, r->nstr, false)) * c->n_cu;
+  r->n_simd = 4 * (int64_t)c->n_cu;
+  r->piece_floor = piece_floor;
   const int64_t N = c->n_cols;
   const int64_t L = o.chunk_len;
   // band scratch only for chunks taller than one band: cap the piece there
   r->chunk_cap = r->multiband ? std::min<int64_t>(N, std::max<int64_t>(o.min_ref_chunk, 65536)) : N;
   const int64_t min_piece = std::min<int64_t>(N, std::max<int64_t>(64, (int64_t)o.min_ref_chunk / 64 * 64));
-  r->items_cap = slots * ((N + min_piece - 1) / min_piece);
+  // the throughput layout's items, or at most wmax items per SIMD (+ one
+  // rounding item per read) for the per-SIMD layouts
+  r->items_cap = slots * ((N + min_piece - 1) / min_piece) + 3 * r->n_simd + slots;
   if (r->multiband) r->scratch_stride = COL_PADL + r->chunk_cap + L + COL_PADR;
   bool ok = hipMalloc(&r->d_state, (size_t)(slots * slot_bytes)) == hipSuccess &&
             hipMalloc(&r->d_stats, (size_t)slots * 2 * sizeof(double)) == hipSuccess &&
