@@ -406,7 +406,10 @@ typedef struct {
   int32_t norm;          /* dtwax_norm_mode, statistics of the read's first chunk */
   double eps;            /* DTWAX_NORM_MEANSTD epsilon */
   float threshold;       /* on min(fwd, rev); <= 0: no early NONTARGET */
-  int32_t min_ref_chunk; /* shortest reference piece per work item, columns (0: 4 * chunk_len) */
+  int32_t min_ref_chunk; /* shortest reference piece per work item, columns; 0: the engine
+                            picks per batch between ~8 pieces per resident wave of at least
+                            4 * chunk_len and one to three pieces per SIMD of at least
+                            2 * chunk_len, whichever its model of the slowest SIMD favours */
   int64_t hbm_budget;    /* bytes of read state at most (0: max_reads slots) */
   float threshold_per_row; /* > 0: length-aware rule, the threshold after k chunks is
                               threshold_per_row * k * chunk_len (replaces threshold) */
