@@ -523,10 +523,45 @@ int dtwax::plan_build(dtwax_ctx* c, const int32_t* h_lens, int32_t n_reads, cons
   // or the band scratch would not fit the budget.
   int64_t chunk = o.ref_chunk ? o.ref_chunk : N;
   if (!o.ref_chunk && !m3) {
-    if (n_units < slots && n_units > 0) {
+    if (n_units < slots && n_units > 0 && p->precision == DTWAX_F16_RENORM) {
+      // the renormalised form's rounding depends on its chunks (each band
+      // subtracts its own item's row minimum): its automatic plans keep
+      // this rule
       int64_t want = (slots + n_units - 1) / n_units;
       int64_t ch = (N + want - 1) / want;
       chunk = std::max<int64_t>(ch, std::min<int64_t>(N, 8 * (int64_t)qmax));
+    } else if (n_units < slots && n_units > 0) {
+      // K equal pieces per read, K by the modelled time of the slowest SIMD:
+      // items per SIMD x (piece + Q - 1) / SIMD throughput at that many
+      // waves.  One binary16 sDTW wave alone on a SIMD does nearly the work
+      // of three sharing it (48.5 kb x 2000-sample reads at S = 32: 1.0 /
+      // 1.05 / 1.06), so a small batch gains from long pieces, one per SIMD,
+      // and loses to pieces that put a second item on some SIMDs.  Measured
+      // against max(N / (slots / n), 8 Q) chunks: 256 reads 4.42 -> 3.47 ms,
+      // 512 reads 8.47 -> 6.39, 1024 reads 12.48 -> 11.69, 2048 reads
+      // 23.42 -> 22.33, 1536 and 2560 reads unchanged.  Binary32 ops issue
+      // side by side only from two waves of a SIMD (a lone wave: about 0.4
+      // of the pair's rate), so binary32 counts 2 for two or more waves.
+      static const double kRateF16[4] = {0.0, 1.0, 1.05, 1.06};
+      static const double kRateF32[4] = {0.0, 1.0, 2.0, 2.0};
+      const double* kRate = p->precision == DTWAX_F32 ? kRateF32 : kRateF16;
+      const int64_t n_simd = 4 * (int64_t)c->n_cu;
+      double t_best = 0.0;
+      int64_t k_best = 1;
+      for (int64_t K = 1; K <= N && n_units * K <= 3 * slots; K++) {
+        const int64_t piece = (N + K - 1) / K;
+        if (K > 1 && piece < 64) break;
+        const int64_t items = n_units * K;
+        const int64_t per_simd = (items + n_simd - 1) / n_simd;
+        const int64_t w = std::min<int64_t>(3, (std::min<int64_t>(items, slots) + n_simd - 1) / n_simd);
+        const int64_t cols = K == 1 ? N : piece + qmax - 1;
+        const double t = (double)per_simd * (double)cols / kRate[w];
+        if (K == 1 || t < t_best) {
+          t_best = t;
+          k_best = K;
+        }
+      }
+      chunk = (N + k_best - 1) / k_best;
     }
     if (multiband) {
       const int64_t budget = scratch_budget();
