@@ -268,6 +268,14 @@ def test_loop_against_simulated_sequencer(sequencer):
     loop = mk.AdaptiveSampling(client, eng, eject="nontarget")
     try:
         loop.run(seconds=4.0)
+        # the actions already sent get their responses before the stream ends
+        # (on a loaded host the last batches can still be in flight)
+        deadline = time.time() + 5.0
+        while time.time() < deadline:
+            got = sum(sum(v.values()) for v in client.responses.values())
+            if client.action_queue.empty() and got >= len(client.sent_actions):
+                break
+            time.sleep(0.02)
     finally:
         client.stop()
     assert client.error is None
@@ -286,9 +294,12 @@ def test_loop_against_simulated_sequencer(sequencer):
     assert all(not t[k] for k in unblocked)
     s = sim.summary()
     assert s["target"]["unblocked"] == 0
-    # chunks reach the engine contiguous and in order: 3200 samples per decided read
+    # every decided read reached the engine with the two chunks its decision
+    # needs; a chunk that arrived while that decision was on its way may have
+    # been submitted too (the real engine answers it STALE, unscored).  Chunk
+    # assembly itself is pinned by test_sample_assembly_and_actions.
     for ch, num, a, chunks, _ in loop.decisions:
-        assert len(eng.chunks[(ch, num)]) == 2
+        assert len(eng.chunks[(ch, num)]) >= 2
 
 
 def test_setup_errors(sequencer):
