@@ -349,7 +349,8 @@ def reference_kernel_leg(ctx, raw_reads, ref_levels, stream, peak_ops):
     with HIP events around each call (untiling, keys, sDTW kernel, dists):
     compat_quirks 1 (the reference kernel's DP, seam deviations included,
     in the persistent kernel), 0 (the clean recurrence), 2 (the deviations
-    from the tile-by-tile kernel)."""
+    from the tile-by-tile kernel); then the same through the FP32 build's
+    mirror (dtwax_distances_f32)."""
     import ctypes as C
     import dtwax_amd as D
     L = D.lib()
@@ -390,6 +391,41 @@ def reference_kernel_leg(ctx, raw_reads, ref_levels, stream, peak_ops):
         out[name] = {"compat_quirks": quirks, "call_ms": round(ms[-1], 3),
                      "gcups": round(cells / (ms[-1] * 1e-3) / 1e9, 1),
                      "roofline_frac": round(3.0 * cells / (ms[-1] * 1e-3) / peak_ops, 4)}
+    # the FP32 build (FullDTW::distances<float>): the same queries against
+    # fwd | rev as 96,256 floats, 115 tiles = 95,680 columns of one sequence
+    cat = np.concatenate([ref_levels[0, :ref_len], ref_levels[1, :ref_len]]).astype(np.float32)
+    n32_tiles = 2 * ref_len // tile
+    c32 = np.arange(n32_tiles * tile)
+    tiled32 = np.zeros(2 * ref_len, np.float32)
+    tiled32[(c32 // tile) * tile + (c32 % tile % seg) * warp + (c32 % tile) // seg] = cat[c32]
+    d_ref32 = D.DeviceArray.from_host(tiled32)
+    d_q32 = D.DeviceArray.from_host(np.concatenate([qs.ravel(), np.zeros(32, np.float32)]))
+    d_out32 = D.DeviceArray((n,), np.float32)
+    cells32 = n * 2048 * n32_tiles * tile
+    fp32, scores32 = {}, {}
+    for name, quirks in (("reference_dp", 1), ("clean", 0), ("reference_dp_tiles", 2)):
+        ms = []
+        for _ in range(2):
+            D._check(L.dtwax_event_record(ev0, stream))
+            ctx.distances_f32(d_ref32, d_q32, d_out32, n, compat_quirks=quirks, stream=stream)
+            D._check(L.dtwax_event_record(ev1, stream))
+            D._check(L.dtwax_stream_sync(stream))
+            v = C.c_float()
+            D._check(L.dtwax_event_elapsed_ms(ev0, ev1, C.byref(v)))
+            ms.append(v.value)
+        scores32[quirks] = d_out32.to_host()
+        fp32[name] = {"compat_quirks": quirks, "call_ms": round(ms[-1], 3),
+                      "gcups": round(cells32 / (ms[-1] * 1e-3) / 1e9, 1),
+                      "roofline_frac": round(3.0 * cells32 / (ms[-1] * 1e-3) / peak_ops, 4)}
+    d32 = scores32[1] - scores32[0]
+    fp32.update({
+        "workload": f"the same {n} queries vs fwd | rev as 96,256 floats, 95,680 columns of one sequence",
+        "tiles_equal_persistent": bool(np.array_equal(scores32[1], scores32[2])),
+        "clean_vs_reference_dp": {"changed_scores": int(np.count_nonzero(d32)),
+                                  "max_abs_diff": float(np.abs(d32).max()),
+                                  "mean_abs_diff": float(f"{np.abs(d32).mean():.4g}")},
+    })
+    out["fp32_build"] = fp32
     L.dtwax_event_destroy(ev0)
     L.dtwax_event_destroy(ev1)
     d = scores[1] - scores[0]

@@ -1013,9 +1013,9 @@ int distances_run(dtwax_ctx* c, const RefBuild& g, uint32_t* rbuf, int32_t* coun
   uint32_t* flat = rbuf + COL_PADL;
   HIP_TRY(launch_untile_reference((const uint32_t*)ref, g.n_tiles, flat, st));
   const int32_t lr_last = g.ref_len - TILE_W;
-  if (compat_quirks == 2 || (compat_quirks == 1 && !g.fp16)) {
-    // the reference's tile-by-tile schedule (the only form with the
-    // deviations for the FP32 build)
+  if (compat_quirks == 2) {
+    // the reference's tile-by-tile schedule: the same bits as 1 from an
+    // independent kernel
     HIP_TRY(launch_compat_tiles(g.fp16, flat, query, num_entries, g.n_tiles, dists, device_last_row, g.ref_len,
                                 lr_last, st));
     return DTWAX_OK;
@@ -1064,7 +1064,7 @@ int distances_impl(dtwax_ctx* c, const RefBuild& g, const void* ref, const void*
     }
     HIP_TRY(launch_fill_u32((uint32_t*)rbuf, total, pad_word(g), st));
   }
-  const bool keyed = !(compat_quirks == 2 || (compat_quirks == 1 && !g.fp16));
+  const bool keyed = compat_quirks != 2;
   if (keyed && c->compat_cap < num_entries) {
     if (c->compat_keys) (void)hipFree(c->compat_keys);
     if (c->compat_counter) (void)hipFree(c->compat_counter);

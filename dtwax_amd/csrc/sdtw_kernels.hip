@@ -1165,15 +1165,18 @@ __global__ __launch_bounds__(64) void sdtw_f16_pack_kernel(KParams p) {
 constexpr int F32_LA = 2;  // cells of look-ahead for the row costs
 template <int S>
 constexpr bool f32_xstep() { return S >= 25; }
-template <int S, int NSTR, bool M3>
+// QX: rows 2..S-1 of the lanes with xdiag set take the next lane's row i - 2
+// of this step as their diagonal (the seam deviation Q2, below).
+template <int S, int NSTR, bool M3, bool QX = false>
 __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)[S],
                                           const float (&s)[NSTR], const float (&s_next)[NSTR],
                                           float (&cn)[NSTR][F32_LA], const float (&tin)[NSTR],
                                           const float (&dg)[NSTR], const float (&Do)[NSTR][S],
-                                          float (&Dn)[NSTR][S]) {
+                                          float (&Dn)[NSTR][S], bool xdiag = false) {
   constexpr int LA = F32_LA;
   constexpr bool XSTEP = f32_xstep<S>();
   static_assert(S >= LA, "look-ahead within one column");
+  static_assert(!QX || (NSTR == 1 && !M3), "seam deviations: one sequence, two moves");
   float tp[NSTR], dd[NSTR], c[NSTR][S];
 #pragma unroll
   for (int st = 0; st < NSTR; st++) {
@@ -1195,8 +1198,18 @@ __device__ __forceinline__ void f32_cells(const float (&A)[S], const float (&Qn)
     float m[NSTR];
 #pragma unroll
     for (int st = 0; st < NSTR; st++) {
-      if constexpr (M3) m[st] = __builtin_fminf(tp[st], __builtin_fminf(dd[st], Do[st][i]));
-      else m[st] = __builtin_fminf(tp[st], dd[st]);
+      if constexpr (M3) {
+        m[st] = __builtin_fminf(tp[st], __builtin_fminf(dd[st], Do[st][i]));
+      } else if constexpr (QX) {
+        float d = dd[st];
+        if (i >= 2) {
+          const float nb = dpp_shl1(d, Dn[st][i - 2]);  // next lane's row i - 2, this step
+          d = xdiag ? nb : d;
+        }
+        m[st] = __builtin_fminf(tp[st], d);
+      } else {
+        m[st] = __builtin_fminf(tp[st], dd[st]);
+      }
     }
     float t[NSTR];
 #pragma unroll
@@ -1248,9 +1261,51 @@ __device__ __forceinline__ void f32_step(const float (&A)[S], const float (&Qn)[
   }
 }
 
-template <int S, int NSTR, bool M3, bool FIX = false>
+// QK: the reference kernel's seam deviations (Q1, Q2 of f16_band2) in the
+// binary32 sweep of its FP32 build: 2048 rows as one band (64 lanes x 32
+// rows) over the one sequence fwd | rev from column 0, one column per step,
+// so lane l reaches column 832k at step 832k + l.  The deviating cells:
+//   Q1  lane 32 row 0: +inf diagonal at steps 832k + 32 (k >= 1) and
+//       832k + 58 (column 832k + 26, k >= 0);
+//   Q2  at steps 832k + l (k >= 2): lanes 31 and 63, row 0, take their own
+//       previous column's row 30; lanes 30 and 62, rows 2..31, take lane
+//       31's (63's) row i - 2 of the same step (f32_cells' QX form).
+// The six steps per tile (t mod 832 = 30, 31, 32, 58, 62, 63) lie in three
+// 8-step groups (24, 32, 56), which run f32_step_qk; the other 101 of every
+// 104 groups run the plain step.
+constexpr int QK32_PERIOD = TILE_W;  // steps per tile (one column per step)
+
+template <int S, bool QX>
+__device__ __forceinline__ void f32_step_qk(const float (&A)[S], const float (&Qn)[S], const float* rs,
+                                            const float* rt, int u, float (&s_cur)[1], float (&tf_cur)[1],
+                                            float (&cn)[1][F32_LA], float (&tin)[1], float (&dg)[1],
+                                            float (&hist)[1], const float (&Do)[1][S], float (&Dn)[1][S],
+                                            int lane, int32_t t, int32_t r) {
+  constexpr int NSTR = 1;
+  float s_next[NSTR], tf_next[NSTR];
+  s_next[0] = rs[u + 1];
+  tf_next[0] = rt[u + 2];
+  const bool q1 = (r == 32 && t >= QK32_PERIOD + 32) || r == 58;
+  const bool q2 = (r == 30 || r == 31 || r == 62 || r == 63) && t >= 2 * QK32_PERIOD + r;
+  float dq[NSTR] = {dg[0]};
+  if (q1 && lane == 32) dq[0] = __builtin_inff();
+  if (q2 && (r & 1) && lane == r) dq[0] = Do[0][S - 2];  // lanes 31, 63: own row 30 of column c - 1
+  f32_cells<S, NSTR, false, QX>(A, Qn, s_cur, s_next, cn, tin, dq, Do, Dn, QX && q2 && lane == r);
+#pragma unroll
+  for (int st = 0; st < NSTR; st++) {
+    dg[st] = tin[st];
+    tin[st] = dpp_shr1(tf_cur[st], Dn[st][S - 1]);
+    hist[st] = dpp_shl1(Dn[st][S - 1], hist[st]);
+    s_cur[st] = s_next[st];
+    tf_cur[st] = tf_next[st];
+  }
+}
+
+template <int S, int NSTR, bool M3, bool FIX = false, bool QK = false>
 __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* ring_s,
                          float* ring_t, float* scratch) {
+  static_assert(!QK || (S == 32 && NSTR == 1 && !M3 && FIX),
+                "seam deviations: binary32 per op, 64 x 32 rows, one sequence from column 0");
   const int32_t read = it.read;
   const int32_t Q = read_length<FIX>(p, read);
   const float* __restrict__ qp = p.queries + read_offset<FIX>(p, read);
@@ -1332,6 +1387,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
         cn[st][i] = t * t;
       }
     }
+    int32_t tmod = 0;  // QK: T mod QK32_PERIOD
     for (int32_t T = 0; T < nsteps; T += 64) {
       float pf_s[NSTR], pf_t[NSTR];
 #pragma unroll
@@ -1344,6 +1400,28 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
       for (int g = 0; g < 64; g += 8) {
         const float* rs = ring_s + (base + g) * NSTR;
         const float* rt = ring_t + (base + g) * NSTR;
+        if constexpr (QK) {
+          // tmod is a multiple of 64 below 832 = 13 x 64, so r0 < 832
+          const int32_t r0 = tmod + g;
+          if (r0 == 24 || r0 == 32 || r0 == 56) {
+            // a group holding a deviating step (uniform across the wave);
+            // only u = 6 of groups 24 and 56 (steps 30, 62) needs the
+            // cross-lane diagonal
+            const int32_t t0 = T + g;
+#pragma unroll
+            for (int u = 0; u < 8; u += 2) {
+              if (u == 6)
+                f32_step_qk<S, true>(A, Qn, rs, rt, u, s_cur, tf_cur, cn, tin, dg, hist, D0, D1, lane, t0 + u,
+                                     r0 + u);
+              else
+                f32_step_qk<S, false>(A, Qn, rs, rt, u, s_cur, tf_cur, cn, tin, dg, hist, D0, D1, lane, t0 + u,
+                                      r0 + u);
+              f32_step_qk<S, false>(A, Qn, rs, rt, u + 1, s_cur, tf_cur, cn, tin, dg, hist, D1, D0, lane,
+                                    t0 + u + 1, r0 + u + 1);
+            }
+            continue;
+          }
+        }
 #pragma unroll
         for (int u = 0; u < 8; u += 2) {
           f32_step<S, NSTR, M3>(A, Qn, rs, rt, u, s_cur, tf_cur, cn, tin, dg, hist, D0, D1);
@@ -1375,6 +1453,10 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
         }
       }
       __builtin_amdgcn_wave_barrier();
+      if constexpr (QK) {
+        tmod += 64;
+        if (tmod >= QK32_PERIOD) tmod -= QK32_PERIOD;
+      }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __threadfence_block();
@@ -1386,7 +1468,7 @@ __device__ void f32_item(const KParams& p, const WorkItem& it, int lane, float* 
   }
 }
 
-template <int S, int NSTR, bool M3, bool FIX = false>
+template <int S, int NSTR, bool M3, bool FIX = false, bool QK = false>
 __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
   __shared__ float ring_s[RING_ALLOC * NSTR];
   __shared__ float ring_t[RING_ALLOC * NSTR];
@@ -1396,7 +1478,7 @@ __global__ __launch_bounds__(64) void sdtw_f32_kernel(KParams p) {
     const int idx = fetch_item(p.counter, lane);
     if (idx >= p.n_items) break;
     WorkItem it = load_item<FIX>(p, idx);
-    f32_item<S, NSTR, M3, FIX>(p, it, lane, ring_s, ring_t, scratch);
+    f32_item<S, NSTR, M3, FIX, QK>(p, it, lane, ring_s, ring_t, scratch);
   }
 }
 
@@ -1525,8 +1607,9 @@ hipError_t launch_sdtw_fixed(const KParams& p, int precision, int grid, bool qui
   if (precision == DTWAX_F16) {
     if (quirks) hipLaunchKernelGGL((sdtw_f16_kernel<32, DTWAX_F16, false, true, true>), dim3(grid), dim3(64), 0, st, p);
     else hipLaunchKernelGGL((sdtw_f16_kernel<32, DTWAX_F16, false, false, true>), dim3(grid), dim3(64), 0, st, p);
-  } else if (precision == DTWAX_F32 && !quirks) {
-    hipLaunchKernelGGL((sdtw_f32_kernel<32, 1, false, true>), dim3(grid), dim3(64), 0, st, p);
+  } else if (precision == DTWAX_F32) {
+    if (quirks) hipLaunchKernelGGL((sdtw_f32_kernel<32, 1, false, true, true>), dim3(grid), dim3(64), 0, st, p);
+    else hipLaunchKernelGGL((sdtw_f32_kernel<32, 1, false, true>), dim3(grid), dim3(64), 0, st, p);
   } else {
     return hipErrorInvalidValue;
   }
@@ -1625,7 +1708,9 @@ int occupancy_f16(int precision, int S, bool m3) {
 
 int occupancy_fixed(int precision, bool quirks) {
   int n = 0;
-  if (precision == DTWAX_F32)
+  if (precision == DTWAX_F32 && quirks)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<32, 1, false, true, true>, 64, 0);
+  else if (precision == DTWAX_F32)
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f32_kernel<32, 1, false, true>, 64, 0);
   else if (quirks)
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sdtw_f16_kernel<32, DTWAX_F16, false, true, true>, 64, 0);

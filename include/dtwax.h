@@ -340,10 +340,11 @@ int dtwax_distances(dtwax_ctx* ctx, const void* ref, const void* query, void* di
  * layout; 115 tiles (95,680 columns) scored as ONE sequence (the DP crosses
  * the fwd -> rev seam); query = num_entries * 2048 floats (+ pad); dists =
  * num_entries floats; device_last_row = NULL or num_entries * REF_LEN floats
- * (tile 114 at REF_LEN - 832).  Binary32 per op.  compat_quirks 1 and 2 both
- * run the tile-by-tile kernel (the deviations are the same as in the FP16
- * build: they come from the schedule, not the type); 0 the clean recurrence
- * on the persistent kernel. */
+ * (tile 114 at REF_LEN - 832).  Binary32 per op.  compat_quirks as for
+ * dtwax_distances: 1 (the reference kernel's bits) on the persistent kernel,
+ * 2 the same bits from the tile-by-tile kernel, 0 the clean recurrence (the
+ * deviations are the same as in the FP16 build: they come from the
+ * schedule, not the type). */
 int dtwax_distances_f32(dtwax_ctx* ctx, const void* ref, const void* query, void* dists,
                         int32_t num_entries, float thresh, void* stream,
                         void* device_last_row, int32_t compat_quirks);
