@@ -190,7 +190,7 @@ def read_until_leg(ctx, seconds, norm="meanstd", burst_seconds=10.0):
                             "fused_form_opt_in": fused_sat},
         "plain_batched_form": plain,
         "serving_vs_plain_reads_per_s": round(m["reads_decided_per_s"] / max(plain["reads_decided_per_s"], 1e-9), 3),
-        "setup_s": round(time.time() - t0, 1),
+        "leg_s": round(time.time() - t0, 1),  # the whole leg: set-up, calibration and every simulation
     })
     if burst_seconds > 0:
         m["burst"] = {"what": "every channel's first read starts at t = 0, so all 512 first chunks arrive "

@@ -54,6 +54,22 @@ PROBE_KERNEL(k_min_i32, OPS2("v_min_i32"))
 PROBE_KERNEL(k_cell_f32_imin, "v_fma_f32 v56, v41, v42, v43\n v_mul_f32 v57, v45, v45\n v_min_i32 v58, v49, v50\n"
                               " v_add_f32 v59, v53, v54\n v_fma_f32 v60, v45, v46, v47\n v_mul_f32 v61, v49, v49\n"
                               " v_min_i32 v62, v51, v52\n v_add_f32 v63, v55, v40\n")
+// other forms of the binary32 min: do any of them pair like add/mul?
+PROBE_KERNEL(k_max_f32, OPS2("v_max_f32"))
+PROBE_KERNEL(k_min3_f32, OPS3("v_min3_f32"))
+PROBE_KERNEL(k_med3_f32, OPS3("v_med3_f32"))
+PROBE_KERNEL(k_minimum3_f32, OPS3("v_minimum3_f32"))
+// the 64-bit (VOP3) encoding of an op that pairs in its 32-bit form
+PROBE_KERNEL(k_add_f32_e64, OPS2("v_add_f32_e64"))
+// a min beside an fma: does the min pair with another kind of op?
+PROBE_KERNEL(k_fma_min_alt, "v_fma_f32 v56, v41, v42, v43\n v_min_f32 v57, v45, v46\n v_fma_f32 v58, v49, v50, v51\n"
+                            " v_min_f32 v59, v53, v54\n v_fma_f32 v60, v45, v46, v47\n v_min_f32 v61, v49, v50\n"
+                            " v_fma_f32 v62, v51, v52, v53\n v_min_f32 v63, v55, v40\n")
+// the binary32 cell with every op in the 32-bit encoding (sub for the row cost)
+PROBE_KERNEL(k_cell_f32_vop2, "v_sub_f32 v56, v41, v42\n v_mul_f32 v57, v45, v45\n v_min_f32 v58, v49, v50\n"
+                              " v_add_f32 v59, v53, v54\n v_sub_f32 v60, v45, v46\n v_mul_f32 v61, v49, v49\n"
+                              " v_min_f32 v62, v51, v52\n v_add_f32 v63, v55, v40\n")
+
 // packed binary16 and binary32 ops alternating (do they issue side by side?)
 PROBE_KERNEL(k_mix_pk_f32, "v_pk_add_f16 v56, v41, v42\n v_add_f32 v57, v45, v46\n v_pk_add_f16 v58, v49, v50\n"
                            " v_add_f32 v59, v53, v54\n v_pk_add_f16 v60, v43, v44\n v_add_f32 v61, v47, v48\n"
@@ -105,7 +121,10 @@ int main(int argc, char** argv) {
       {"v_mul_f32", k_mul_f32, 1.0},       {"v_fma_f32", k_fma_f32, 1.0},       {"v_add_f16", k_add_f16, 1.0},
       {"v_min_i32", k_min_i32, 1.0},       {"f32_cell_mix_imin", k_cell_f32_imin, 1.0},
       {"f32_cell_mix", k_cell_f32, 1.0},   {"pk_f16+f32_alt", k_mix_pk_f32, 1.5}, {"v_mov_b32_dpp", k_dpp, 1.0},
-      {"7_pk_f16+1_dpp", k_pk_dpp, 14.0 / 8.0}};
+      {"7_pk_f16+1_dpp", k_pk_dpp, 14.0 / 8.0}, {"v_max_f32", k_max_f32, 1.0}, {"v_min3_f32", k_min3_f32, 1.0},
+      {"v_med3_f32", k_med3_f32, 1.0},     {"v_minimum3_f32", k_minimum3_f32, 1.0},
+      {"v_add_f32_e64", k_add_f32_e64, 1.0}, {"fma+min_alt", k_fma_min_alt, 1.0},
+      {"f32_cell_mix_vop2", k_cell_f32_vop2, 1.0}};
   const int nprobe = (int)(sizeof(probes) / sizeof(probes[0]));
   const int max_waves = 4;
   double rate[4][32];
