@@ -447,8 +447,9 @@ def reference_kernel_leg(ctx, raw_reads, ref_levels, stream, peak_ops):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): the path's sharding over "
+              "several GPUs (reads or reference chunks per GPU, one min-reduction) is not built",
+              file=sys.stderr)
         sys.exit(2)
     if a.workload == "c4":
         return main_c4(a)
