@@ -90,6 +90,9 @@ struct KParams {
   const PackLane* pack_lanes;  // 64 per pack (packed launches)
   const PackRead* pack_reads;
   int32_t scratch_rows;        // band scratch rows per workgroup (packed launches; else 1)
+  // binary16 sweeps: nonzero lowers each wave's issue priority as it moves
+  // through a band (set when the launch has more waves than the GPU has SIMDs)
+  int32_t prio_by_progress;
   // Fixed-geometry kernels (launch_sdtw_fixed): read i is q_stride samples
   // from i * q_stride (q_off, q_len and items unused).
   int32_t q_stride;

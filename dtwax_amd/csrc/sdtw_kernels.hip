@@ -91,6 +91,28 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// Issue priority by progress.  A SIMD's VALU issues from its oldest ready
+// wave first, so of the two or three sDTW waves sharing a SIMD the oldest
+// runs nearly alone and the youngest barely moves until the others are done
+// (measured per item: 11.7 / 22.2 / 32.8 ms to finish three equal items
+// started together).  At the end of a launch that youngest wave then runs
+// its last item alone, below the SIMD's shared rate, about 1 ms per launch.
+// The binary16 sweeps therefore lower their priority as they move through a
+// band (3 in its first quarter down to 0 in its last): whichever wave is
+// behind issues first, and the waves of a SIMD finish together (48.5 kb x
+// 4096 x 2000: 43.9 -> 43.2 ms; fused 34.0 -> 32.9, renormalised 46.5 ->
+// 44.8).  Launches with no more waves than SIMDs leave it off (a Read Until
+// chunk's 1024 pieces ran 1-2% slower with it).  The binary32 sweep keeps
+// the default order: its ops pair only across two waves, and the same rule
+// lost 9% there on 4096 reads.
+__device__ __forceinline__ void progress_prio(int32_t T, int32_t nsteps) {
+  const int q = (int)(((int64_t)T * 4) / (nsteps > 0 ? nsteps : 1));
+  if (q <= 0) __builtin_amdgcn_s_setprio(3);
+  else if (q == 1) __builtin_amdgcn_s_setprio(2);
+  else if (q == 2) __builtin_amdgcn_s_setprio(1);
+  else __builtin_amdgcn_s_setprio(0);
+}
+
 // Work fetch and result publication without lane-divergent branches: every
 // lane issues the atomic, lane 0 with the real operand and the others with
 // the identity, and the wave takes lane 0's result.  (A lane-0-only branch
@@ -258,6 +280,7 @@ struct SweepIO {
   int32_t lr_ntiles, lr_last;
   h2 tlow;           // DTWAX_F16_RENORM: subtracted from the top row as it is staged
                      // (the previous band's, or the stored row's, exact minimum)
+  bool prio;         // issue priority by progress through the band (progress_prio)
 };
 
 // Sweep one band across columns [x, c1) (+ ramp).  Accumulates the min/argmin
@@ -320,6 +343,7 @@ __device__ __forceinline__ void f16_sweep(const F16Band<S, MODE>& B, const Sweep
   for (int i = 0; i < PG; i++) tc[i] = hfma2(s_cur, B.A[i], B.Qn[i]);
 
   for (int32_t T = 0; T < nsteps; T += 64) {
+    if (io.prio) progress_prio(T, nsteps);
     // Stream block T+128 .. T+191 (written to the ring after this block).
     h2 pf_s = ref[x + T + 128 + lane];
     h2 pf_t = trow ? trow[x + T + 128 + lane] : ZERO;
@@ -584,6 +608,7 @@ __device__ __forceinline__ void f16_band2(const F16Band<S, MODE>& B, const Sweep
   const int32_t nsteps = (((c1 - x + 127) >> 1) + 63) & ~63;
   int32_t tmod = 0;  // QK: T mod QK_PERIOD
   for (int32_t T = 0; T < nsteps; T += 64) {
+    if (io.prio) progress_prio(T, nsteps);
     // offsets [2T + 130, 2T + 258): written to the ring after this block
     h2 pf_s[2], pf_t[2];
 #pragma unroll
@@ -758,6 +783,7 @@ __device__ __forceinline__ void f16_band2_pack(const F16Band<S, MODE>& B, const 
 
   const int32_t nsteps = (((c1 - x + 127) >> 1) + 63) & ~63;
   for (int32_t T = 0; T < nsteps; T += 64) {
+    if (io.prio) progress_prio(T, nsteps);
     h2 pf_s[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) pf_s[h] = ref[x + 2 * T + 130 + 64 * h + lane];
@@ -867,6 +893,7 @@ __device__ void f16_item_split(const KParams& p, const WorkItem& it, int lane, h
     constexpr int SB = decltype(width)::value;
     SweepIO io;
     io.ref = (const h2*)p.ref;
+    io.prio = p.prio_by_progress != 0;
     h2* scr = scratch - it.x;  // indexed by absolute column
     io.trow = (b == 0) ? nullptr : scr;
     io.final_band = (b == nb - 1);
@@ -978,6 +1005,7 @@ __device__ void f16_item(const KParams& p, const WorkItem& it, int lane, h2* rin
     }
     SweepIO io;
     io.ref = (const h2*)p.ref;
+    io.prio = p.prio_by_progress != 0;
     // the band scratch row is indexed by absolute column; it holds [x - 64, c1 + 384)
     h2* scr = scratch - it.x;
     io.trow = (b == 0) ? trow_ext : scr;
@@ -1057,6 +1085,7 @@ __device__ void f16_pack_item(const KParams& p, const WorkItem& it, int lane, h2
   const PackLane pl = p.pack_lanes[64 * (int64_t)it.read + lane];
   SweepIO io;
   io.ref = (const h2*)p.ref;
+  io.prio = p.prio_by_progress != 0;
   io.trow = nullptr;
   io.orow = nullptr;
   io.x = it.x;
