@@ -897,7 +897,7 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
       k.pack_lanes = L.pack_lanes;
       k.pack_reads = L.pack_reads;
       const int grid = std::max(1, std::min(G.grid, G.n_items));
-      k.prio_by_progress = grid > 4 * c->n_cu ? 1 : 0;  // some SIMD runs two or more of its waves
+      k.prio_by_progress = (L.prio_ok && grid > 4 * c->n_cu) ? 1 : 0;  // some SIMD runs two or more of its waves
       if (L.fixed) HIP_TRY(launch_sdtw_fixed(k, L.precision, grid, L.quirks, gs));
       else HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, grid, L.m3, gs, G.pack));
     }

@@ -58,6 +58,12 @@ struct ScoreLaunch {
   int64_t row_stride = 0;
   float renorm_k = 0.f;
   bool m3 = false;  // three moves
+  // binary16 sweeps may order a SIMD's waves by progress (KParams::
+  // prio_by_progress) when the launch has more waves than SIMDs; the Read
+  // Until engine turns this off: its batches share SIMDs with each other, and
+  // a later batch's high-priority waves held back earlier ones (decision
+  // latency p99 6.0-6.5 -> 6.9-8.0 ms on a 512-channel flow cell)
+  bool prio_ok = true;
   DeviceResult* out = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   const PackLane* pack_lanes = nullptr;  // packed groups' lane and read tables

@@ -101,8 +101,9 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 // band (3 in its first quarter down to 0 in its last): whichever wave is
 // behind issues first, and the waves of a SIMD finish together (48.5 kb x
 // 4096 x 2000: 43.9 -> 43.2 ms; fused 34.0 -> 32.9, renormalised 46.5 ->
-// 44.8).  Launches with no more waves than SIMDs leave it off (a Read Until
-// chunk's 1024 pieces ran 1-2% slower with it).  The binary32 sweep keeps
+// 44.8).  Launches with no more waves than SIMDs, and the Read Until
+// engine's batches, which share SIMDs with each other, leave it off.  The
+// binary32 sweep keeps
 // the default order: its ops pair only across two waves, and the same rule
 // lost 9% there on 4096 reads.
 __device__ __forceinline__ void progress_prio(int32_t T, int32_t nsteps) {

@@ -389,6 +389,7 @@ int launch(dtwax_ru* r, Lane& l, const std::vector<int32_t>& channels, const std
   sl.out = l.d_res;
   sl.ev0 = l.ev0;
   sl.ev1 = l.ev1;
+  sl.prio_ok = false;  // batches run side by side: default wave order
   int rc = run_score_launch(c, sl, l.st);
   if (rc) return rc;
   DTWAX_HIP_TRY(hipMemcpyAsync(l.h_res, l.d_res, (size_t)m * sizeof(DeviceResult), hipMemcpyDeviceToHost, l.st));
