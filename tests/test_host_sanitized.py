@@ -9,9 +9,11 @@ tests/golden/make_fuzz_corpus.py): every file either reads or is reported
 and skipped, with no sanitizer report; the unchanged control files' reads
 come through whole.  Malformed command lines exit with status 2, cleanly.
 """
+import fcntl
 import os
 import subprocess
 import sys
+import tempfile
 
 import pytest
 
@@ -28,7 +30,11 @@ ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
 
 @pytest.fixture(scope="module")
 def asan_bin():
-    r = subprocess.run(["make", "-C", CSRC, "asan"], capture_output=True, text=True, timeout=600)
+    # parallel test workers each reach this fixture: one builds (or relinks)
+    # while the others wait, so none runs a half-written binary
+    with open(os.path.join(tempfile.gettempdir(), "dtwax_asan_build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        r = subprocess.run(["make", "-C", CSRC, "asan"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     return ASAN_BIN
 
