@@ -882,6 +882,13 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
   // units to the next width instead of idling them.
   if (n_groups > 1) HIP_TRY(hipEventRecord(L.fork, st));
   const int64_t epc = (L.precision == DTWAX_F32) ? L.nstr : 1;
+  // progress-ordered waves when some SIMD runs two or more of this
+  // launch's waves; one setting for every group, so that groups running
+  // side by side share the SIMDs on equal terms
+  bool prio = false;
+  for (int g = 0; g < n_groups; g++)
+    if (groups[g].n_items > 0 && std::min(groups[g].grid, groups[g].n_items) > 4 * c->n_cu) prio = true;
+  k.prio_by_progress = (L.prio_ok && prio) ? 1 : 0;
   for (int g = 0; g < n_groups; g++) {
     const LaunchGroup& G = groups[g];
     hipStream_t gs = (g == 0 || !G.stream) ? st : G.stream;
@@ -897,7 +904,6 @@ int run_score_launch(dtwax_ctx* c, const ScoreLaunch& L, hipStream_t st) {
       k.pack_lanes = L.pack_lanes;
       k.pack_reads = L.pack_reads;
       const int grid = std::max(1, std::min(G.grid, G.n_items));
-      k.prio_by_progress = (L.prio_ok && grid > 4 * c->n_cu) ? 1 : 0;  // some SIMD runs two or more of its waves
       if (L.fixed) HIP_TRY(launch_sdtw_fixed(k, L.precision, grid, L.quirks, gs));
       else HIP_TRY(launch_sdtw(k, L.precision, G.S, L.nstr, grid, L.m3, gs, G.pack));
     }

@@ -91,7 +91,8 @@ struct KParams {
   const PackRead* pack_reads;
   int32_t scratch_rows;        // band scratch rows per workgroup (packed launches; else 1)
   // binary16 sweeps: nonzero lowers each wave's issue priority as it moves
-  // through a band (set when the launch has more waves than the GPU has SIMDs)
+  // through a band (set for every group of a launch when one of them has
+  // more waves than the GPU has SIMDs)
   int32_t prio_by_progress;
   // Fixed-geometry kernels (launch_sdtw_fixed): read i is q_stride samples
   // from i * q_stride (q_off, q_len and items unused).
