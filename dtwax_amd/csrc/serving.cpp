@@ -518,7 +518,9 @@ void worker_main(dtwax_ru* r) {
       if (take.empty() && !progressed) {
         if (r->stop_req && r->inq.empty() && !any_busy) idle_exit = true;
         else if (!any_busy && r->inq.empty()) r->cv_in.wait(lk, [&] { return r->stop_req || !r->inq.empty(); });
-        else r->cv_in.wait_for(lk, std::chrono::microseconds(50));
+        // a batch is on the GPU: look again soon (a 10 us poll instead of
+        // 50 us took ~25 us off the decision p50 on a 512-channel flow cell)
+        else r->cv_in.wait_for(lk, std::chrono::microseconds(10));
       }
     }
     if (idle_exit) return;
