@@ -17,6 +17,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <vector>
+
 #define R8(x) x x x x x x x x
 #define PROBE_CLOBBER                                                                                       \
   "v40", "v41", "v42", "v43", "v44", "v45", "v46", "v47", "v48", "v49", "v50", "v51", "v52", "v53", "v54", \
@@ -127,7 +129,8 @@ int main(int argc, char** argv) {
       {"f32_cell_mix_vop2", k_cell_f32_vop2, 1.0}};
   const int nprobe = (int)(sizeof(probes) / sizeof(probes[0]));
   const int max_waves = 4;
-  double rate[4][32];
+  std::vector<double> rate_v((size_t)max_waves * nprobe);  // rate(w - 1, k): w waves per SIMD, probe k
+  auto rate = [&](int w1, int k) -> double& { return rate_v[(size_t)w1 * nprobe + k]; };
   float* out = nullptr;
   CHECK(hipMalloc(&out, (size_t)ncu * 4 * max_waves * 64 * sizeof(float)));
   hipEvent_t a, b;
@@ -153,8 +156,8 @@ int main(int argc, char** argv) {
       float ms = 0.0f;
       CHECK(hipEventElapsedTime(&ms, a, b));
       const double instr = (double)blocks * iters * 64.0;
-      rate[w - 1][k] = instr / ncu / ((double)ms * 1e-3 * clk_khz * 1e3);
-      printf("  %13.3f", rate[w - 1][k]);
+      rate(w - 1, k) = instr / ncu / ((double)ms * 1e-3 * clk_khz * 1e3);
+      printf("  %13.3f", rate(w - 1, k));
     }
     printf("   %.1f\n", probes[k].elems);
   }
@@ -163,7 +166,7 @@ int main(int argc, char** argv) {
          prop.gcnArchName, ncu, clk_khz / 1000);
   for (int k = 0; k < nprobe; k++) {
     printf("%s\"%s\": [", k ? ", " : "", probes[k].name);
-    for (int w = 1; w <= max_waves; w++) printf("%s%.3f", w > 1 ? ", " : "", rate[w - 1][k]);
+    for (int w = 1; w <= max_waves; w++) printf("%s%.3f", w > 1 ? ", " : "", rate(w - 1, k));
     printf("]");
   }
   printf("}}\n");
