@@ -139,6 +139,12 @@ def test_distances_f32_build(ctx):
     # and above 0 where their path crosses a deviating cell
     assert np.all(got[2:] == 0)
     assert want[3] == 0 and np.all(want[2:][np.arange(len(starts)) != 1] > 0)
+    # launches of one and three queries (grids of one and three waves), no last row
+    for m in (1, 3):
+        d_sub = D.DeviceArray((m,), np.float32)
+        ctx.distances_f32(d_ref, d_q, d_sub, m, compat_quirks=1)
+        D.lib().dtwax_device_sync()
+        assert np.array_equal(d_sub.to_host(), want[:m]), m
 
 
 def test_distances_keeps_uploaded_reference_two_streams(ctx):
