@@ -92,7 +92,13 @@ struct Lane {
   unsigned long long* d_keys = nullptr;
   DeviceResult* d_res = nullptr;
   void* d_scratch = nullptr;
-  uint32_t* d_rn = nullptr;  // DTWAX_F16_RENORM: {K, k0} half2 words per read
+  // One pinned host block and one device block stage a batch's inputs, laid
+  // out per batch (slots, first flags, renorm words, items, raw samples) so
+  // that a single copy moves them; the pointers below are views into them.
+  unsigned char* h_stage = nullptr;
+  unsigned char* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  uint32_t* d_rn = nullptr;  // DTWAX_F16_RENORM: {K, k0} half2 words per read (else null)
   uint32_t* h_rn = nullptr;
   float* h_raw = nullptr;
   int32_t* h_slot = nullptr;
@@ -164,11 +170,10 @@ namespace {
 
 void lane_free(Lane& l) {
   if (l.st) (void)hipStreamSynchronize(l.st);
-  for (void* p : {(void*)l.d_raw, (void*)l.d_norm, (void*)l.d_slot, (void*)l.d_first, (void*)l.d_items,
-                  (void*)l.d_counter, (void*)l.d_keys, (void*)l.d_res, l.d_scratch, (void*)l.d_rn})
+  for (void* p : {(void*)l.d_stage, (void*)l.d_norm, (void*)l.d_counter, (void*)l.d_keys, (void*)l.d_res,
+                  l.d_scratch})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)l.h_raw, (void*)l.h_slot, (void*)l.h_first, (void*)l.h_items, (void*)l.h_res,
-                  (void*)l.h_rn})
+  for (void* p : {(void*)l.h_stage, (void*)l.h_res})
     if (p) (void)hipHostFree(p);
   for (hipEvent_t e : {l.ev0, l.ev1, l.done})
     if (e) (void)hipEventDestroy(e);
@@ -176,27 +181,24 @@ void lane_free(Lane& l) {
   l = Lane{};
 }
 
+constexpr size_t kStageAlign = 256;
+size_t stage_part(size_t bytes) { return (bytes + kStageAlign - 1) / kStageAlign * kStageAlign; }
+
 bool lane_alloc(dtwax_ru* r, Lane& l) {
   const size_t slots = (size_t)r->slots, L = (size_t)r->o.chunk_len;
   bool ok = hipStreamCreate(&l.st) == hipSuccess && hipEventCreate(&l.ev0) == hipSuccess &&
             hipEventCreate(&l.ev1) == hipSuccess &&
             hipEventCreateWithFlags(&l.done, hipEventDisableTiming) == hipSuccess;
-  ok = ok && hipMalloc(&l.d_raw, slots * L * sizeof(float)) == hipSuccess &&
+  // the largest batch's stage: every part rounded up to kStageAlign
+  l.stage_bytes = stage_part(slots * sizeof(int32_t)) * 2 + stage_part(slots * 2 * sizeof(uint32_t)) +
+                  stage_part((size_t)r->items_cap * sizeof(WorkItem)) + stage_part(slots * L * sizeof(float));
+  ok = ok && hipMalloc(&l.d_stage, l.stage_bytes) == hipSuccess &&
        hipMalloc(&l.d_norm, slots * L * sizeof(float)) == hipSuccess &&
-       hipMalloc(&l.d_slot, slots * sizeof(int32_t)) == hipSuccess &&
-       hipMalloc(&l.d_first, slots * sizeof(int32_t)) == hipSuccess &&
-       hipMalloc(&l.d_items, (size_t)r->items_cap * sizeof(WorkItem)) == hipSuccess &&
        hipMalloc(&l.d_counter, sizeof(int32_t)) == hipSuccess &&
        hipMalloc(&l.d_keys, slots * 2 * sizeof(unsigned long long)) == hipSuccess &&
        hipMalloc(&l.d_res, slots * sizeof(DeviceResult)) == hipSuccess;
-  ok = ok && hipHostMalloc((void**)&l.h_raw, slots * L * sizeof(float), 0) == hipSuccess &&
-       hipHostMalloc((void**)&l.h_slot, slots * sizeof(int32_t), 0) == hipSuccess &&
-       hipHostMalloc((void**)&l.h_first, slots * sizeof(int32_t), 0) == hipSuccess &&
-       hipHostMalloc((void**)&l.h_items, (size_t)r->items_cap * sizeof(WorkItem), 0) == hipSuccess &&
+  ok = ok && hipHostMalloc((void**)&l.h_stage, l.stage_bytes, 0) == hipSuccess &&
        hipHostMalloc((void**)&l.h_res, slots * sizeof(DeviceResult), 0) == hipSuccess;
-  if (ok && r->o.precision == DTWAX_F16_RENORM)
-    ok = hipMalloc(&l.d_rn, slots * 2 * sizeof(uint32_t)) == hipSuccess &&
-         hipHostMalloc((void**)&l.h_rn, slots * 2 * sizeof(uint32_t), 0) == hipSuccess;
   if (ok && r->multiband) {
     const int64_t n = r->grid_slots * r->scratch_stride * r->epc;
     ok = hipMalloc(&l.d_scratch, (size_t)n * 4) == hipSuccess &&
@@ -327,6 +329,30 @@ int launch(dtwax_ru* r, Lane& l, const std::vector<int32_t>& channels, const std
   const int64_t piece = piece_for(r, m);
   const int64_t per_read = (N + piece - 1) / piece;
   if (m > r->slots || m * per_read > r->items_cap) return set_error(DTWAX_EINVAL, "batch capacity exceeded");
+  // this batch's stage layout; the views point into the lane's two blocks
+  const bool rn = r->o.precision == DTWAX_F16_RENORM;
+  size_t off = 0;
+  auto part = [&](size_t bytes) {
+    const size_t o = off;
+    off += stage_part(bytes);
+    return o;
+  };
+  const size_t o_slot = part((size_t)m * sizeof(int32_t));
+  const size_t o_first = part((size_t)m * sizeof(int32_t));
+  const size_t o_rn = part(rn ? (size_t)m * 2 * sizeof(uint32_t) : 0);
+  const size_t o_items = part((size_t)(m * per_read) * sizeof(WorkItem));
+  const size_t o_raw = part((size_t)(m * L) * sizeof(float));
+  if (off > l.stage_bytes) return set_error(DTWAX_EINVAL, "batch stage exceeds its buffer");
+  l.h_slot = (int32_t*)(l.h_stage + o_slot);
+  l.h_first = (int32_t*)(l.h_stage + o_first);
+  l.h_rn = rn ? (uint32_t*)(l.h_stage + o_rn) : nullptr;
+  l.h_items = (WorkItem*)(l.h_stage + o_items);
+  l.h_raw = (float*)(l.h_stage + o_raw);
+  l.d_slot = (int32_t*)(l.d_stage + o_slot);
+  l.d_first = (int32_t*)(l.d_stage + o_first);
+  l.d_rn = rn ? (uint32_t*)(l.d_stage + o_rn) : nullptr;
+  l.d_items = (WorkItem*)(l.d_stage + o_items);
+  l.d_raw = (float*)(l.d_stage + o_raw);
   int64_t ni = 0;
   for (int64_t j = 0; j < m; j++) {
     const ChanRec& rec = r->chans[channels[(size_t)j]];
@@ -359,12 +385,8 @@ int launch(dtwax_ru* r, Lane& l, const std::vector<int32_t>& channels, const std
     }
   }
   DTWAX_HIP_TRY(hipSetDevice(c->device));
-  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_raw, l.h_raw, (size_t)(m * L) * sizeof(float), hipMemcpyHostToDevice, l.st));
-  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_slot, l.h_slot, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, l.st));
-  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_first, l.h_first, (size_t)m * sizeof(int32_t), hipMemcpyHostToDevice, l.st));
-  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_items, l.h_items, (size_t)ni * sizeof(WorkItem), hipMemcpyHostToDevice, l.st));
-  if (l.d_rn)
-    DTWAX_HIP_TRY(hipMemcpyAsync(l.d_rn, l.h_rn, (size_t)m * 2 * sizeof(uint32_t), hipMemcpyHostToDevice, l.st));
+  // one copy for the whole stage (the raw samples last: it ends with them)
+  DTWAX_HIP_TRY(hipMemcpyAsync(l.d_stage, l.h_stage, off, hipMemcpyHostToDevice, l.st));
   DTWAX_HIP_TRY(launch_ru_normalize(l.d_raw, (int32_t)m, L, l.d_slot, l.d_first, r->o.norm, r->o.eps,
                                     r->d_stats, l.d_norm, l.st));
   ScoreLaunch sl;
